@@ -1,0 +1,193 @@
+"""Benchmark of the BPC-PaCo codec on one MI355X.
+
+One step = encode + decode of the workload's images, with coefficients, codestreams
+and LUTs resident in HBM.  The default workload is the plain per-image launch on a
+4096x4096 image (4096 codeblocks of 64x64, 5-level DWT, 16 magnitude bitplanes,
+3 coding passes), synthetic data drawn as the workload definition states.
+
+Prints one JSON line (metric, value in Msamples/s, roofline of the dominant kernel,
+CPU baseline from the oracle on the host cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cuda_bpc-paco_amd"))
+
+import bpcpaco as bpc  # noqa: E402
+import synth  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "encode+decode Msamples/s (4096² 5-lvl DWT), batched/served vs plain; p50/p99"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_workload(size, cp3, nbp, mag_bits, ll_bits, detail_exp, seed, levels=5):
+    img = synth.synth_image(size, size, levels, seed, ll_bits=ll_bits, detail_exp=detail_exp, mag_bits=mag_bits)
+    train = synth.synth_image(size, size, levels, seed + 1, ll_bits=ll_bits, detail_exp=detail_exp,
+                              mag_bits=mag_bits)
+    p = bpc.params(cpass3=cp3, dwt_levels=levels, lut_n_bitplanes=nbp, layout=bpc.LAYOUT_TILED)
+    lut = bpc.train_lut(train, bpc.params(cpass3=cp3, dwt_levels=levels, lut_n_bitplanes=nbp))
+    return img, lut, p
+
+
+def cpu_baseline(img, lut_arr, p, cp3, nbp, budget_s=20.0):
+    """Oracle (C, OpenMP over codeblocks) encode+decode of the same image on the host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    op = O.params(cpass3=cp3, dwt_levels=p.dwt_levels, lut_n_bitplanes=nbp)
+    Y, X = img.shape
+    reps, total, samples = 0, 0.0, 0
+    enc_stream = None
+    while total < budget_s and reps < 50:
+        t0 = time.perf_counter()
+        st, stream, ncw = O.encode(img, lut_arr, op, threads)
+        t1 = time.perf_counter()
+        st2, rec = O.decode(stream, X, Y, lut_arr, op, threads)
+        t2 = time.perf_counter()
+        assert st == 0 and st2 == 0
+        total += t2 - t0
+        samples += img.size
+        reps += 1
+        enc_stream = stream
+    return {"value": samples / total / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": "%d x encode+decode of the full %dx%d image (oracle, OpenMP over codeblocks)" % (reps, X, Y)}, \
+        enc_stream
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the oracle baseline and parity check")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every codeblock is independent and a multi-GPU run "
+              "would be independent replicas; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    if bpc.device_count() < 1:
+        print("no HIP device", file=sys.stderr)
+        sys.exit(1)
+
+    size, cp3, nbp = 4096, 1, 16
+    log("workload plain_4096_3pass: generating image and training LUT")
+    img, lut_arr, p = make_workload(size, cp3, nbp, mag_bits=16, ll_bits=13, detail_exp=9, seed=args.seed)
+    X = Y = size
+    ncb = p.n_codeblocks(X, Y)
+    lut = bpc.Lut.from_array(lut_arr)
+    stream = bpc.Stream()
+    d_in = bpc.DeviceBuffer.from_array(synth.to_tiled(img))
+    nw = bpc.stream_words(X, Y, p)
+    d_cs = bpc.DeviceBuffer(nw)
+    d_out = bpc.DeviceBuffer(X * Y)
+    d_ncw = bpc.DeviceBuffer(ncb)
+    d_status = bpc.DeviceBuffer(1, np.uint32)
+    d_status.fill_bytes(0)
+    bpc.device_sync()  # setup copies and the memset ran on the null stream
+    ev = [bpc.Event() for _ in range(4)]
+
+    def step(timed_pairs=None):
+        ev[0].record(stream)
+        bpc.encode_device(d_in, d_cs, X, Y, lut, p, 1, d_ncw, d_status, stream)
+        ev[1].record(stream)
+        ev[2].record(stream)
+        bpc.decode_device(d_cs, d_out, X, Y, lut, p, d_status, stream)
+        ev[3].record(stream)
+        if timed_pairs is not None:
+            timed_pairs.append((ev[0].elapsed_ms(ev[1]), ev[2].elapsed_ms(ev[3])))
+
+    log("warmup %d steps" % args.warmup)
+    for _ in range(args.warmup):
+        step()
+    rc = bpc.status_take(d_status, stream)
+    if rc != 0:
+        raise RuntimeError("codec status %d: %s" % (rc, bpc.lib().bpc_last_error().decode()))
+    # correctness of this run: round trip, and the codestream against the oracle below
+    rec = synth.from_tiled(d_out.download(stream=stream), X, Y)
+    if not np.array_equal(rec, img):
+        raise RuntimeError("decode(encode(x)) != x")
+    ncw = d_ncw.download(stream=stream)
+    gstream = d_cs.download(stream=stream)
+
+    # timed region: K steps between two full synchronisations
+    bpc.device_sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    stream.sync()
+    bpc.device_sync()
+    wall = time.perf_counter() - t0
+    # per-kernel durations from HIP events on the launch stream (separate pass, same work)
+    pairs = []
+    for _ in range(args.steps):
+        step(pairs)
+    rc = bpc.status_take(d_status, stream)
+    if rc != 0:
+        raise RuntimeError("codec status %d" % rc)
+    enc_ms = float(np.mean([a for a, _ in pairs]))
+    dec_ms = float(np.mean([b for _, b in pairs]))
+    enc_p50, enc_p99 = np.percentile([a for a, _ in pairs], [50, 99])
+    dec_p50, dec_p99 = np.percentile([b for _, b in pairs], [50, 99])
+
+    samples = X * Y
+    ms_per_step = wall * 1e3 / args.steps
+    value = samples * args.steps / wall / 1e6
+    # algorithmic bytes per direction: 16384 B of coefficients + 4*(1 + n_cw) B of
+    # codestream words (MSB + used slots) per codeblock
+    alg_bytes = float(ncb * 16384 + 4 * (ncb + int(ncw.sum())))
+    dom_name, dom_ms = ("decoder", dec_ms) if dec_ms >= enc_ms else ("encoder", enc_ms)
+    achieved = alg_bytes / (dom_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded Laplacian/uniform DWT coefficients, LUT trained on a second seeded image)",
+        "config": {"workload": "plain_4096_3pass", "mode": "plain per-image launch", "image": "4096x4096",
+                   "codeblocks": ncb, "codeblock": "64x64", "dwt_levels": 5, "cpass3": cp3,
+                   "magnitude_bitplanes": 16, "lut_n_bitplanes": nbp, "layout": "codeblock-contiguous",
+                   "seed": args.seed},
+        "encode_ms": round(enc_ms, 4), "decode_ms": round(dec_ms, 4),
+        "encode_ms_p50_p99": [round(float(enc_p50), 4), round(float(enc_p99), 4)],
+        "decode_ms_p50_p99": [round(float(dec_p50), 4), round(float(dec_p99), 4)],
+        "bits_per_sample": round(float(ncw.sum()) * 16 / samples, 4),
+        "roofline": {"bound": "hbm", "kernel": "bpc_codec_kernel (%s)" % dom_name,
+                     "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                     "algorithmic_bytes_per_launch": alg_bytes},
+        "cpu_baseline": None,
+        "bit_exact_vs_oracle": None,
+    }
+    if not args.no_cpu:
+        log("cpu baseline (oracle)")
+        cb, ostream = cpu_baseline(img, lut_arr, p, cp3, nbp)
+        out["cpu_baseline"] = cb
+        out["bit_exact_vs_oracle"] = bool(np.array_equal(gstream, ostream))
+        if not out["bit_exact_vs_oracle"]:
+            raise RuntimeError("GPU codestream differs from the oracle")
+    print(json.dumps(out), flush=True)
+    for b in (d_in, d_cs, d_out, d_ncw, d_status):
+        b.free()
+    lut.free()
+
+
+if __name__ == "__main__":
+    main()

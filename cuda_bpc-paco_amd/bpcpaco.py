@@ -1,0 +1,369 @@
+"""Python mirror of the BPC-PaCo C-ABI (libbpc_paco.so, declared in include/bpc_paco.h).
+
+The reference's host interface is
+
+    void CUDA_BPC(int Direction, DATATYPE* HData, int HDSize_X, int HDSize_Y, int* LUT)
+    void load_LUTs(int** LUT, const char* folder)
+
+(source/BPC_PaCo.cu:1305, source/main_example.cu:219).  ``run`` and
+``Lut.from_text`` mirror them: same direction codes (CODE=0, DECODE=1), same
+in-place buffer semantics, same LUT layout.  Errors raise ``BPCError`` with the
+library's status code instead of exiting the process (common.cu:3-8).
+
+Everything here calls the HIP library; there is no CPU fallback.  Loading fails
+loudly when libbpc_paco.so has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbpc_paco.so")
+
+CODE = 0
+DECODE = 1
+LAYOUT_RASTER = 0
+LAYOUT_TILED = 1
+
+OK = 0
+ERR_ARG = -1
+ERR_HIP = -2
+ERR_OVERFLOW = -3
+ERR_RANGE = -4
+ERR_MALFORMED = -5
+ERR_WIDE = -6
+ERR_NODEVICE = -7
+
+
+class BPCError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("BPC status %d: %s" % (code, msg))
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    """bpc_params: runtime form of CPASS_3, DWT_LEVELS, CBLOCK_LENGTH, LUT_N_BITPLANES (common.h)."""
+    _fields_ = [
+        ("cpass3", ctypes.c_int32),
+        ("dwt_levels", ctypes.c_int32),
+        ("cblock_length", ctypes.c_int32),
+        ("lut_n_bitplanes", ctypes.c_int32),
+        ("cb_stride_words", ctypes.c_int32),
+        ("layout", ctypes.c_int32),
+    ]
+
+    def stride(self):
+        return self.cb_stride_words if self.cb_stride_words > 0 else 64 * self.cblock_length
+
+    def n_codeblocks(self, size_x, size_y):
+        return (size_x // 64) * (size_y // self.cblock_length)
+
+
+def params(cpass3=0, dwt_levels=5, cblock_length=64, lut_n_bitplanes=15, cb_stride_words=0, layout=LAYOUT_RASTER):
+    return Params(cpass3, dwt_levels, cblock_length, lut_n_bitplanes, cb_stride_words, layout)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libbpc_paco.so is not built (run __graft_entry__.build() or make -C cuda_bpc-paco_amd)")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        i32 = ctypes.c_int
+        PP = ctypes.POINTER(Params)
+        sz = ctypes.c_size_t
+        sig = {
+            "bpc_default_params": (None, [PP]),
+            "bpc_lut_len": (i32, [PP]),
+            "bpc_stream_words": (ctypes.c_int64, [i32, i32, PP]),
+            "bpc_last_error": (ctypes.c_char_p, []),
+            "bpc_device_count": (i32, []),
+            "bpc_run": (i32, [i32, P, i32, i32, P]),
+            "bpc_set_config": (i32, [PP]),
+            "bpc_get_config": (i32, [PP]),
+            "bpc_run_ex": (i32, [i32, P, i32, i32, P, i32, PP]),
+            "bpc_lut_load_text": (i32, [ctypes.c_char_p, i32, i32, ctypes.POINTER(P), ctypes.POINTER(i32),
+                                        ctypes.POINTER(i32)]),
+            "bpc_lut_parse_text": (i32, [ctypes.c_char_p, i32, i32, P, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]),
+            "bpc_lut_upload": (i32, [P, i32, ctypes.POINTER(P)]),
+            "bpc_lut_free": (i32, [P]),
+            "bpc_lut_train": (i32, [P, i32, i32, PP, P, i32, i32]),
+            "bpc_encode_device": (i32, [P, P, i32, i32, P, i32, PP, i32, P, P, P]),
+            "bpc_decode_device": (i32, [P, P, i32, i32, P, i32, PP, P, P]),
+            "bpc_status_take": (i32, [P, P]),
+            "bpc_dev_malloc": (i32, [ctypes.POINTER(P), sz]),
+            "bpc_dev_free": (i32, [P]),
+            "bpc_dev_memset": (i32, [P, i32, sz, P]),
+            "bpc_copy_h2d": (i32, [P, P, sz, P]),
+            "bpc_copy_d2h": (i32, [P, P, sz, P]),
+            "bpc_stream_create": (i32, [ctypes.POINTER(P)]),
+            "bpc_stream_destroy": (i32, [P]),
+            "bpc_stream_sync": (i32, [P]),
+            "bpc_device_sync": (i32, []),
+            "bpc_event_create": (i32, [ctypes.POINTER(P)]),
+            "bpc_event_destroy": (i32, [P]),
+            "bpc_event_record": (i32, [P, P]),
+            "bpc_event_elapsed_ms": (i32, [P, P, ctypes.POINTER(ctypes.c_float)]),
+            "bpc_mem_info": (i32, [ctypes.POINTER(sz), ctypes.POINTER(sz)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(code):
+    if code != OK:
+        raise BPCError(code, lib().bpc_last_error().decode(errors="replace"))
+    return code
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def lut_len(p):
+    return lib().bpc_lut_len(ctypes.byref(p))
+
+
+def stream_words(size_x, size_y, p):
+    return lib().bpc_stream_words(size_x, size_y, ctypes.byref(p))
+
+
+def device_count():
+    return lib().bpc_device_count()
+
+
+# ---- device memory -----------------------------------------------------------------
+
+class DeviceBuffer:
+    """A hipMalloc'ed buffer of n elements of dtype."""
+
+    def __init__(self, n, dtype=np.int32):
+        self.dtype = np.dtype(dtype)
+        self.n = int(n)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        _check(lib().bpc_dev_malloc(ctypes.byref(p), max(1, self.nbytes)))
+        self.ptr = p
+
+    @classmethod
+    def from_array(cls, arr, stream=None):
+        arr = np.ascontiguousarray(arr)
+        b = cls(arr.size, arr.dtype)
+        b.upload(arr, stream)
+        return b
+
+    def upload(self, arr, stream=None):
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert arr.size <= self.n
+        _check(lib().bpc_copy_h2d(self.ptr, _ptr(arr), arr.nbytes, _sp(stream)))
+
+    def download(self, n=None, stream=None):
+        n = self.n if n is None else n
+        out = np.empty(n, dtype=self.dtype)
+        _check(lib().bpc_copy_d2h(_ptr(out), self.ptr, out.nbytes, _sp(stream)))
+        return out
+
+    def fill_bytes(self, value, stream=None):
+        _check(lib().bpc_dev_memset(self.ptr, value, self.nbytes, _sp(stream)))
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            lib().bpc_dev_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        p = ctypes.c_void_p()
+        _check(lib().bpc_stream_create(ctypes.byref(p)))
+        self.ptr = p
+
+    def sync(self):
+        _check(lib().bpc_stream_sync(self.ptr))
+
+    def destroy(self):
+        if self.ptr is not None:
+            lib().bpc_stream_destroy(self.ptr)
+            self.ptr = None
+
+
+class Event:
+    def __init__(self):
+        p = ctypes.c_void_p()
+        _check(lib().bpc_event_create(ctypes.byref(p)))
+        self.ptr = p
+
+    def record(self, stream=None):
+        _check(lib().bpc_event_record(self.ptr, _sp(stream)))
+
+    def elapsed_ms(self, later):
+        ms = ctypes.c_float()
+        _check(lib().bpc_event_elapsed_ms(self.ptr, later.ptr, ctypes.byref(ms)))
+        return ms.value
+
+    def destroy(self):
+        if self.ptr is not None:
+            lib().bpc_event_destroy(self.ptr)
+            self.ptr = None
+
+
+def _sp(stream):
+    if stream is None:
+        return None
+    return stream.ptr if isinstance(stream, Stream) else stream
+
+
+def device_sync():
+    _check(lib().bpc_device_sync())
+
+
+def mem_info():
+    f, t = ctypes.c_size_t(), ctypes.c_size_t()
+    _check(lib().bpc_mem_info(ctypes.byref(f), ctypes.byref(t)))
+    return f.value, t.value
+
+
+# ---- probability LUTs --------------------------------------------------------------
+
+class Lut:
+    """A device-resident probability LUT in the reference layout (main_example.cu:219-448)."""
+
+    def __init__(self, dev_ptr, length, nbp=None, host=None):
+        self.ptr = dev_ptr
+        self.len = length
+        self.nbp = nbp
+        self.host = host
+
+    @classmethod
+    def from_text(cls, folder, cpass3, dwt_levels=5):
+        """load_LUTs (main_example.cu:219): parse ref/sig/sign(/cp_sig/cp_sign).txt and upload."""
+        p = ctypes.c_void_p()
+        ln, nbp = ctypes.c_int(), ctypes.c_int()
+        _check(lib().bpc_lut_load_text(folder.encode(), cpass3, dwt_levels, ctypes.byref(p), ctypes.byref(ln),
+                                       ctypes.byref(nbp)))
+        return cls(p, ln.value, nbp.value)
+
+    @classmethod
+    def from_array(cls, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.int32)
+        p = ctypes.c_void_p()
+        _check(lib().bpc_lut_upload(_ptr(arr), arr.size, ctypes.byref(p)))
+        return cls(p, arr.size, host=arr)
+
+    def free(self):
+        if self.ptr is not None:
+            lib().bpc_lut_free(self.ptr)
+            self.ptr = None
+
+
+def parse_lut_text(folder, cpass3, dwt_levels=5):
+    """Host-side parse of the LUT text files; returns (int32 array, nbp)."""
+    cap = 1 << 16
+    out = np.zeros(cap, dtype=np.int32)
+    ln, nbp = ctypes.c_int(), ctypes.c_int()
+    _check(lib().bpc_lut_parse_text(folder.encode(), cpass3, dwt_levels, _ptr(out), cap, ctypes.byref(ln),
+                                    ctypes.byref(nbp)))
+    return out[: ln.value].copy(), nbp.value
+
+
+def train_lut(img, p, nthreads=0):
+    """Probability LUT from symbol frequencies of a raster training image."""
+    img = np.ascontiguousarray(img, dtype=np.int32)
+    Y, X = img.shape
+    n = lut_len(p)
+    out = np.zeros(n, dtype=np.int32)
+    _check(lib().bpc_lut_train(_ptr(img), X, Y, ctypes.byref(p), _ptr(out), n, nthreads))
+    return out
+
+
+# ---- codec -------------------------------------------------------------------------
+
+def run(direction, hdata, size_x, size_y, lut, p=None):
+    """CUDA_BPC (BPC_PaCo.cu:1305): hdata (host int32 numpy array) is overwritten in place.
+
+    CODE: coefficients -> codestream regions; DECODE: regions -> coefficients.
+    With p=None the library's current configuration (bpc_set_config) is used."""
+    assert hdata.dtype == np.int32 and hdata.flags["C_CONTIGUOUS"]
+    if p is None:
+        _check(lib().bpc_run(direction, _ptr(hdata), size_x, size_y, lut.ptr))
+    else:
+        need = max(size_x * size_y, stream_words(size_x, size_y, p))
+        assert hdata.size >= need, "hdata must hold %d int32" % need
+        _check(lib().bpc_run_ex(direction, _ptr(hdata), size_x, size_y, lut.ptr, lut.len, ctypes.byref(p)))
+    return hdata
+
+
+def set_config(p):
+    _check(lib().bpc_set_config(ctypes.byref(p)))
+
+
+def get_config():
+    p = Params()
+    _check(lib().bpc_get_config(ctypes.byref(p)))
+    return p
+
+
+def encode_device(d_coeffs, d_stream, size_x, size_y, lut, p, fill_unused=1, d_ncw=None, d_status=None, stream=None):
+    _check(lib().bpc_encode_device(d_coeffs.ptr, d_stream.ptr, size_x, size_y, lut.ptr, lut.len, ctypes.byref(p),
+                                   fill_unused, d_ncw.ptr if d_ncw is not None else None,
+                                   d_status.ptr if d_status is not None else None, _sp(stream)))
+
+
+def decode_device(d_stream, d_coeffs, size_x, size_y, lut, p, d_status=None, stream=None):
+    _check(lib().bpc_decode_device(d_stream.ptr, d_coeffs.ptr, size_x, size_y, lut.ptr, lut.len, ctypes.byref(p),
+                                   d_status.ptr if d_status is not None else None, _sp(stream)))
+
+
+def status_take(d_status=None, stream=None):
+    """Synchronizes and returns the BPC status code of the launches since the last call (no raise)."""
+    return lib().bpc_status_take(d_status.ptr if d_status is not None else None, _sp(stream))
+
+
+def encode(img, lut, p, fill_unused=1):
+    """Convenience: upload, encode on the device, download.  Returns (stream words, ncw per codeblock)."""
+    Y, X = img.shape if p.layout == LAYOUT_RASTER else _tiled_shape(img, p)
+    d_in = DeviceBuffer.from_array(np.ascontiguousarray(img, dtype=np.int32).ravel())
+    nw = stream_words(X, Y, p)
+    d_out = DeviceBuffer(nw)
+    ncb = p.n_codeblocks(X, Y)
+    d_ncw = DeviceBuffer(ncb)
+    try:
+        encode_device(d_in, d_out, X, Y, lut, p, fill_unused, d_ncw)
+        _check(status_take())
+        return d_out.download(), d_ncw.download()
+    finally:
+        d_in.free()
+        d_out.free()
+        d_ncw.free()
+
+
+def decode(stream_words_arr, size_x, size_y, lut, p):
+    d_in = DeviceBuffer.from_array(np.ascontiguousarray(stream_words_arr, dtype=np.int32))
+    d_out = DeviceBuffer(size_x * size_y)
+    try:
+        decode_device(d_in, d_out, size_x, size_y, lut, p)
+        _check(status_take())
+        out = d_out.download()
+    finally:
+        d_in.free()
+        d_out.free()
+    return out.reshape(size_y, size_x) if p.layout == LAYOUT_RASTER else out
+
+
+def _tiled_shape(img, p):
+    raise ValueError("tiled input needs explicit sizes; use encode_device")

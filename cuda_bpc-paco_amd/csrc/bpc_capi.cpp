@@ -1,0 +1,623 @@
+// C-ABI of the BPC-PaCo codec: drop-in run entry (CUDA_BPC, BPC_PaCo.cu:1305),
+// device-resident encode/decode, LUT loading / upload / training, HIP helpers.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <atomic>
+#include <mutex>
+#include <thread>
+#include <string>
+#include <vector>
+
+#include "../../include/bpc_paco.h"
+#include "bpc_internal.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(BPC_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr)                                   \
+  do {                                                  \
+    hipError_t e_ = (expr);                             \
+    if (e_ != hipSuccess) return hip_fail(e_, #expr);   \
+  } while (0)
+
+std::mutex g_cfg_mu;
+bpc_params g_cfg = {0, 5, 64, 15, 0, BPC_LAYOUT_RASTER};
+
+int stride_of(const bpc_params* p) { return p->cb_stride_words > 0 ? p->cb_stride_words : 64 * p->cblock_length; }
+
+int check_params(const bpc_params* p) {
+  if (!p) return fail(BPC_ERR_ARG, "null params");
+  if (p->cpass3 != 0 && p->cpass3 != 1) return fail(BPC_ERR_ARG, "cpass3 must be 0 or 1");
+  if (p->dwt_levels < 0 || p->dwt_levels > 16) return fail(BPC_ERR_ARG, "dwt_levels out of range");
+  if (p->cblock_length < 2 || p->cblock_length > 64) return fail(BPC_ERR_ARG, "cblock_length must be 2..64");
+  if (p->lut_n_bitplanes < 1 || p->lut_n_bitplanes > 28) return fail(BPC_ERR_ARG, "lut_n_bitplanes must be 1..28");
+  if (p->cb_stride_words < 0 || (p->cb_stride_words > 0 && p->cb_stride_words < 2))
+    return fail(BPC_ERR_ARG, "cb_stride_words must be 0 or >= 2");
+  if (p->layout != BPC_LAYOUT_RASTER && p->layout != BPC_LAYOUT_TILED) return fail(BPC_ERR_ARG, "bad layout");
+  return BPC_OK;
+}
+
+int check_image(int size_x, int size_y, const bpc_params* p) {
+  if (size_x <= 0 || size_y <= 0 || size_x % 64 != 0 || size_y % p->cblock_length != 0)
+    return fail(BPC_ERR_ARG, "image %dx%d is not a whole number of 64x%d codeblocks", size_x, size_y,
+                p->cblock_length);
+  if ((int64_t)size_x * size_y > (int64_t)1 << 31) return fail(BPC_ERR_ARG, "image too large");
+  return BPC_OK;
+}
+
+int status_to_code(uint32_t st) {
+  if (st & bpc::ST_MALFORMED) return fail(BPC_ERR_MALFORMED, "decoder: malformed MSB word in a codeblock region");
+  if (st & bpc::ST_RANGE)
+    return fail(BPC_ERR_RANGE, "coefficient magnitude >= 2^28, LUT entry outside [0,127] or LUT row outside the array");
+  if (st & bpc::ST_OVERFLOW) return fail(BPC_ERR_OVERFLOW, "codestream region capacity exceeded");
+  if (st & bpc::ST_WIDE) return fail(BPC_ERR_WIDE, "codeblock MSB beyond the kernel's bitplane registers");
+  return BPC_OK;
+}
+
+// one library-owned status word per device
+std::mutex g_status_mu;
+std::vector<uint32_t*> g_status_words;
+
+int default_status(uint32_t** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_status_mu);
+  if ((int)g_status_words.size() <= dev) g_status_words.resize(dev + 1, nullptr);
+  if (!g_status_words[dev]) {
+    uint32_t* w = nullptr;
+    HIP_TRY(hipMalloc(&w, sizeof(uint32_t)));
+    HIP_TRY(hipMemset(w, 0, sizeof(uint32_t)));
+    g_status_words[dev] = w;
+  }
+  *out = g_status_words[dev];
+  return BPC_OK;
+}
+
+int launch(int decode, const int32_t* src, int32_t* dst, int size_x, int size_y, const int32_t* d_lut, int lut_len,
+           const bpc_params* p, int fill, int32_t* d_ncw, uint32_t* d_status, hipStream_t stream, int maxp) {
+  bpc::KArgs a;
+  a.src = src;
+  a.dst = dst;
+  a.lut = d_lut;
+  a.ncw = d_ncw;
+  a.status = d_status;
+  a.size_x = size_x;
+  a.size_y = size_y;
+  a.nx = size_x / 64;
+  a.ncb = (size_x / 64) * (size_y / p->cblock_length);
+  a.rows = p->cblock_length;
+  a.stride = stride_of(p);
+  a.nbp = p->lut_n_bitplanes;
+  a.levels = p->dwt_levels;
+  a.lut_len = lut_len;
+  a.layout = p->layout;
+  a.fill = fill;
+  hipError_t e = bpc::launch_codec(decode, p->cpass3, maxp, a, stream);
+  if (e != hipSuccess) return hip_fail(e, "kernel launch");
+  return BPC_OK;
+}
+
+int check_device_call(const void* a, const void* b, int size_x, int size_y, const int32_t* d_lut, int lut_len,
+                      const bpc_params* p) {
+  int st = check_params(p);
+  if (st) return st;
+  st = check_image(size_x, size_y, p);
+  if (st) return st;
+  if (!a || !b || !d_lut) return fail(BPC_ERR_ARG, "null buffer");
+  if (((uintptr_t)a & 7) || ((uintptr_t)b & 7)) return fail(BPC_ERR_ARG, "buffers must be 8-byte aligned");
+  if (lut_len < bpc_lut_len(p)) return fail(BPC_ERR_ARG, "lut_len %d < %d required", lut_len, bpc_lut_len(p));
+  return BPC_OK;
+}
+
+// ---- LUT text format (main_example.cu:219-448) ----------------------------
+
+struct LutRow {
+  int bp;
+  int v[9];
+};
+
+int read_lut_file(const std::string& path, int nvals, std::vector<LutRow>& rows) {
+  FILE* f = fopen(path.c_str(), "rb");
+  if (!f) return fail(BPC_ERR_ARG, "cannot open %s", path.c_str());
+  int lvl, sb, bp;
+  char colon[4];
+  while (fscanf(f, "%d %d %d %3s", &lvl, &sb, &bp, colon) == 4) {
+    LutRow r;
+    r.bp = bp;
+    for (int k = 0; k < nvals; ++k)
+      if (fscanf(f, "%d", &r.v[k]) != 1) {
+        fclose(f);
+        return fail(BPC_ERR_ARG, "malformed row in %s", path.c_str());
+      }
+    rows.push_back(r);
+  }
+  fclose(f);
+  return BPC_OK;
+}
+
+// Region fill with the loader's rule: a row whose bitplane is <= the previous
+// one starts the next group; before moving on, (maxbp - prev)*N entries from
+// i + prev*N + 1 are set to 64 (main_example.cu:292-299 and its siblings).
+int place_region(const std::vector<LutRow>& rows, int N, int maxbp, int base, int32_t* lut, int cap) {
+  int i = base, prev = -1;
+  for (const LutRow& r : rows) {
+    if (r.bp <= prev) {
+      for (int z = 0; z < (maxbp - prev) * N; ++z) {
+        const int idx = i + prev * N + z + 1;
+        if (idx >= 0 && idx < cap) lut[idx] = 64;
+      }
+      i += maxbp * N;
+    }
+    prev = r.bp;
+    for (int k = 0; k < N; ++k) {
+      const int idx = i + r.bp * N + k;
+      if (idx < 0 || idx >= cap) return fail(BPC_ERR_ARG, "LUT file row outside the table");
+      lut[idx] = r.v[k];
+    }
+  }
+  return BPC_OK;
+}
+
+// ---- LUT trainer --------------------------------------------------------
+
+constexpr uint32_t S_BIT = 0x80000000u, C_BIT = 0x40000000u, R_BIT = 0x20000000u;
+
+struct Counts {
+  // [region][group][bitplane][context] -> (n0, n)
+  std::vector<uint32_t> n0, n;
+};
+
+int group_of(int x, int y, int X, int Y, int levels) {
+  for (int aux = 1; aux <= levels; ++aux) {
+    const bool bx = x >= (X >> aux), by = y >= (Y >> aux);
+    if (bx || by) return (aux - 1) * 3 + (bx ? (by ? 2 : 0) : 1);
+  }
+  return levels * 3;
+}
+
+int chi_of(uint32_t w) { return (w >> 31) == 0 ? 0 : ((w & 1) ? -1 : 1); }
+
+int sign_ctx(int h, int v) {
+  if (h == 0) return v == 0 ? 0 : (v > 0 ? 2 : 3);
+  if (h > 0) return v == 0 ? 4 : (v > 0 ? 6 : 0);
+  return v == 0 ? 5 : (v > 0 ? 1 : 7);
+}
+
+// Models one codeblock with the coder's pass and scan order (rows top to bottom,
+// even columns then odd columns of each row) and counts the coded symbols.
+void train_codeblock(const int32_t* img, int X, int Y, int cb, const bpc_params* p, int G, int nbp,
+                     uint32_t* n0, uint32_t* n) {
+  const int rows = p->cblock_length;
+  const int nx = X / 64;
+  const int tx = cb % nx, ty = cb / nx;
+  static thread_local std::vector<uint32_t> wbuf;
+  wbuf.assign((size_t)rows * 64, 0);
+  uint32_t* w = wbuf.data();
+  uint32_t m = 0;
+  for (int r = 0; r < rows; ++r)
+    for (int c = 0; c < 64; ++c) {
+      const int32_t v = img[(int64_t)(ty * rows + r) * X + tx * 64 + c];
+      const uint32_t mag = v < 0 ? 0u - (uint32_t)v : (uint32_t)v;
+      if (mag >= (1u << 28)) return;
+      m |= mag;
+      w[r * 64 + c] = (mag << 1) | (v < 0 ? 1u : 0u) | (p->cpass3 ? C_BIT : 0u);
+    }
+  if (m == 0) return;
+  int msb = 31;
+  while (!(m >> msb)) --msb;
+  int grp[64];
+  for (int c = 0; c < 64; c += 2) grp[c] = grp[c + 1] = group_of(tx * 64 + c, ty * rows, X, Y, p->dwt_levels);
+  // region order in the count arrays: 0 ref(1), 1 sig(9), 2 sign(4), 3 cp_sig(9), 4 cp_sign(4)
+  auto count = [&](int region, int g, int bp, int ctx, uint32_t sym) {
+    if (bp >= nbp) return;
+    const size_t idx = (((size_t)region * G + g) * nbp + bp) * 9 + ctx;
+    n[idx]++;
+    if (!sym) n0[idx]++;
+  };
+  auto at = [&](int r, int c) -> uint32_t {
+    if (r < 0 || r >= rows || c < 0 || c >= 64) return 0u;
+    return w[r * 64 + c];
+  };
+  auto sig_pass = [&](int bp, bool clean) {
+    for (int r = 0; r < rows; ++r)
+      for (int par = 0; par < 2; ++par)
+        for (int c = par; c < 64; c += 2) {
+          uint32_t& x = w[r * 64 + c];
+          const uint32_t nb[8] = {at(r - 1, c - 1), at(r - 1, c), at(r - 1, c + 1), at(r, c - 1),
+                                  at(r, c + 1),     at(r + 1, c - 1), at(r + 1, c), at(r + 1, c + 1)};
+          int ctx = 0;
+          for (int k = 0; k < 8; ++k) ctx += (int)(nb[k] >> 31);
+          if (!clean) {
+            if (x & S_BIT) continue;
+            if (p->cpass3 && ctx == 0) {
+              x |= C_BIT;
+              continue;
+            }
+          } else {
+            if (!(x & C_BIT)) continue;
+            x &= ~C_BIT;
+          }
+          const uint32_t sym = (x >> (bp + 1)) & 1u;
+          count(clean ? 3 : 1, grp[c], bp, ctx, sym);
+          if (sym) {
+            x |= S_BIT;
+            if (clean) x |= R_BIT;
+            const int sc = sign_ctx(chi_of(nb[3]) + chi_of(nb[4]), chi_of(nb[1]) + chi_of(nb[6]));
+            count(clean ? 4 : 2, grp[c], bp, sc >> 1, ((x & 1u) == (uint32_t)(sc & 1)) ? 0u : 1u);
+          }
+        }
+  };
+  auto ref_pass = [&](int bp) {
+    for (int r = 0; r < rows; ++r)
+      for (int par = 0; par < 2; ++par)
+        for (int c = par; c < 64; c += 2) {
+          uint32_t& x = w[r * 64 + c];
+          if (x & R_BIT)
+            count(0, grp[c], bp, 0, (x >> (bp + 1)) & 1u);
+          else if (x & S_BIT)
+            x |= R_BIT;
+        }
+  };
+  int bp = msb;
+  if (p->cpass3) {
+    sig_pass(bp, true);
+    bp--;
+  }
+  for (; bp >= 0; --bp) {
+    sig_pass(bp, false);
+    ref_pass(bp);
+    if (p->cpass3) sig_pass(bp, true);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+void bpc_default_params(bpc_params* p) {
+  if (!p) return;
+  p->cpass3 = 0;
+  p->dwt_levels = 5;
+  p->cblock_length = 64;
+  p->lut_n_bitplanes = 15;
+  p->cb_stride_words = 0;
+  p->layout = BPC_LAYOUT_RASTER;
+}
+
+int bpc_lut_len(const bpc_params* p) {
+  if (!p) return 0;
+  const int G = 3 * p->dwt_levels + 1;
+  return G * p->lut_n_bitplanes * (1 + 9 + 4 + (p->cpass3 ? 9 + 4 : 0));
+}
+
+int64_t bpc_stream_words(int size_x, int size_y, const bpc_params* p) {
+  if (!p || p->cblock_length <= 0 || size_x <= 0 || size_y <= 0) return 0;
+  return (int64_t)(size_x / 64) * (size_y / p->cblock_length) * stride_of(p);
+}
+
+const char* bpc_last_error(void) { return g_last_error.c_str(); }
+
+int bpc_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int bpc_set_config(const bpc_params* p) {
+  int st = check_params(p);
+  if (st) return st;
+  std::lock_guard<std::mutex> lk(g_cfg_mu);
+  g_cfg = *p;
+  return BPC_OK;
+}
+
+int bpc_get_config(bpc_params* p) {
+  if (!p) return fail(BPC_ERR_ARG, "null params");
+  std::lock_guard<std::mutex> lk(g_cfg_mu);
+  *p = g_cfg;
+  return BPC_OK;
+}
+
+int bpc_encode_device(const int32_t* d_coeffs, int32_t* d_stream, int size_x, int size_y, const int32_t* d_lut,
+                      int lut_len, const bpc_params* p, int fill_unused, int32_t* d_ncw, uint32_t* d_status,
+                      void* stream) {
+  int st = check_device_call(d_coeffs, d_stream, size_x, size_y, d_lut, lut_len, p);
+  if (st) return st;
+  if (!d_status) {
+    st = default_status(&d_status);
+    if (st) return st;
+  }
+  return launch(0, d_coeffs, d_stream, size_x, size_y, d_lut, lut_len, p, fill_unused, d_ncw, d_status,
+                (hipStream_t)stream, bpc::planes_for(p->lut_n_bitplanes));
+}
+
+int bpc_decode_device(const int32_t* d_stream, int32_t* d_coeffs, int size_x, int size_y, const int32_t* d_lut,
+                      int lut_len, const bpc_params* p, uint32_t* d_status, void* stream) {
+  int st = check_device_call(d_stream, d_coeffs, size_x, size_y, d_lut, lut_len, p);
+  if (st) return st;
+  if (!d_status) {
+    st = default_status(&d_status);
+    if (st) return st;
+  }
+  return launch(1, d_stream, d_coeffs, size_x, size_y, d_lut, lut_len, p, 0, nullptr, d_status,
+                (hipStream_t)stream, bpc::planes_for(p->lut_n_bitplanes));
+}
+
+int bpc_status_take(uint32_t* d_status, void* stream) {
+  if (!d_status) {
+    int st = default_status(&d_status);
+    if (st) return st;
+  }
+  uint32_t h = 0;
+  HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+  HIP_TRY(hipMemcpy(&h, d_status, sizeof h, hipMemcpyDeviceToHost));
+  if (h) HIP_TRY(hipMemset(d_status, 0, sizeof h));
+  return status_to_code(h);
+}
+
+int bpc_run_ex(int direction, int32_t* hdata, int size_x, int size_y, const int32_t* d_lut, int lut_len,
+               const bpc_params* p) {
+  if (direction != BPC_CODE && direction != BPC_DECODE) return fail(BPC_ERR_ARG, "direction must be CODE or DECODE");
+  int st = check_params(p);
+  if (st) return st;
+  st = check_image(size_x, size_y, p);
+  if (st) return st;
+  if (!hdata || !d_lut) return fail(BPC_ERR_ARG, "null buffer");
+  if (lut_len < bpc_lut_len(p)) return fail(BPC_ERR_ARG, "lut_len %d < %d required", lut_len, bpc_lut_len(p));
+  if (bpc_device_count() <= 0) return fail(BPC_ERR_NODEVICE, "no HIP device");
+  const size_t img_bytes = (size_t)size_x * size_y * sizeof(int32_t);
+  const size_t cs_bytes = (size_t)bpc_stream_words(size_x, size_y, p) * sizeof(int32_t);
+  const size_t in_bytes = direction == BPC_CODE ? img_bytes : cs_bytes;
+  const size_t out_bytes = direction == BPC_CODE ? cs_bytes : img_bytes;
+  void *d_in = nullptr, *d_out = nullptr;
+  uint32_t* d_status = nullptr;
+  int rc = BPC_OK;
+  hipError_t e = hipMalloc(&d_in, in_bytes);
+  if (e == hipSuccess) e = hipMalloc(&d_out, out_bytes);
+  if (e == hipSuccess) e = hipMalloc(&d_status, sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMemset(d_status, 0, sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMemcpy(d_in, hdata, in_bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) rc = hip_fail(e, "device buffers");
+  int maxp = bpc::planes_for(p->lut_n_bitplanes);
+  for (int attempt = 0; rc == BPC_OK && attempt < 2; ++attempt) {
+    rc = launch(direction == BPC_DECODE, (const int32_t*)d_in, (int32_t*)d_out, size_x, size_y, d_lut, lut_len, p, 1,
+                nullptr, d_status, nullptr, maxp);
+    if (rc) break;
+    uint32_t h = 0;
+    e = hipMemcpy(&h, d_status, sizeof h, hipMemcpyDeviceToHost);  // synchronizes the launch
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "kernel");
+      break;
+    }
+    if ((h & bpc::ST_WIDE) && maxp < 28) {  // rerun with every bitplane register
+      maxp = 28;
+      e = hipMemset(d_status, 0, sizeof(uint32_t));
+      if (e != hipSuccess) rc = hip_fail(e, "hipMemset");
+      continue;
+    }
+    rc = status_to_code(h);
+    break;
+  }
+  if (rc == BPC_OK) {
+    e = hipMemcpy(hdata, d_out, out_bytes, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = hip_fail(e, "copy back");
+  }
+  if (d_in) (void)hipFree(d_in);
+  if (d_out) (void)hipFree(d_out);
+  if (d_status) (void)hipFree(d_status);
+  return rc;
+}
+
+int bpc_run(int direction, int32_t* hdata, int size_x, int size_y, const int32_t* d_lut) {
+  bpc_params p;
+  bpc_get_config(&p);
+  return bpc_run_ex(direction, hdata, size_x, size_y, d_lut, bpc_lut_len(&p), &p);
+}
+
+int bpc_lut_parse_text(const char* folder, int cpass3, int dwt_levels, int32_t* host_lut, int cap, int* lut_len,
+                       int* nbp) {
+  if (!folder || !host_lut || cap <= 0) return fail(BPC_ERR_ARG, "bad arguments");
+  if (cpass3 != 0 && cpass3 != 1) return fail(BPC_ERR_ARG, "cpass3 must be 0 or 1");
+  std::string dir(folder);
+  if (!dir.empty() && dir.back() != '/') dir += '/';
+  const char* names[5] = {"ref.txt", "sig.txt", "sign.txt", "cp_sig.txt", "cp_sign.txt"};
+  const int nv[5] = {1, 9, 4, 9, 4};
+  std::vector<LutRow> rows[5];
+  const int nfiles = cpass3 ? 5 : 3;
+  for (int k = 0; k < nfiles; ++k) {
+    int st = read_lut_file(dir + names[k], nv[k], rows[k]);
+    if (st) return st;
+  }
+  int maxbp = -1;  // taken over ref, sig and sign (main_example.cu:231-264)
+  for (int k = 0; k < 3; ++k)
+    for (const LutRow& r : rows[k]) maxbp = std::max(maxbp, r.bp);
+  maxbp++;
+  if (maxbp < 1) return fail(BPC_ERR_ARG, "empty LUT files");
+  const int G = 3 * dwt_levels + 1;
+  const int refN = G * maxbp, sigN = G * maxbp * 9, signN = G * maxbp * 4;
+  const int total = refN + sigN + signN + (cpass3 ? sigN + signN : 0);
+  if (total > cap) return fail(BPC_ERR_ARG, "LUT needs %d ints, capacity %d", total, cap);
+  for (int k = 0; k < cap; ++k) host_lut[k] = 64;
+  const int bases[5] = {0, refN, refN + sigN, refN + sigN + signN, refN + 2 * sigN + signN};
+  for (int k = 0; k < nfiles; ++k) {
+    int st = place_region(rows[k], nv[k], maxbp, bases[k], host_lut, cap);
+    if (st) return st;
+  }
+  if (lut_len) *lut_len = total;
+  if (nbp) *nbp = maxbp;
+  return BPC_OK;
+}
+
+int bpc_lut_upload(const int32_t* host_lut, int lut_len, int32_t** d_lut) {
+  if (!host_lut || lut_len <= 0 || !d_lut) return fail(BPC_ERR_ARG, "bad arguments");
+  for (int k = 0; k < lut_len; ++k)
+    if (host_lut[k] < 0 || host_lut[k] > 127) return fail(BPC_ERR_RANGE, "LUT entry %d = %d outside [0,127]", k, host_lut[k]);
+  int32_t* d = nullptr;
+  HIP_TRY(hipMalloc(&d, (size_t)lut_len * sizeof(int32_t)));
+  hipError_t e = hipMemcpy(d, host_lut, (size_t)lut_len * sizeof(int32_t), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(d);
+    return hip_fail(e, "LUT upload");
+  }
+  *d_lut = d;
+  return BPC_OK;
+}
+
+int bpc_lut_load_text(const char* folder, int cpass3, int dwt_levels, int32_t** d_lut, int* lut_len, int* nbp) {
+  std::vector<int32_t> h(1 << 16);
+  int len = 0, n = 0;
+  int st = bpc_lut_parse_text(folder, cpass3, dwt_levels, h.data(), (int)h.size(), &len, &n);
+  if (st) return st;
+  st = bpc_lut_upload(h.data(), len, d_lut);
+  if (st) return st;
+  if (lut_len) *lut_len = len;
+  if (nbp) *nbp = n;
+  return BPC_OK;
+}
+
+int bpc_lut_free(int32_t* d_lut) {
+  if (d_lut) HIP_TRY(hipFree(d_lut));
+  return BPC_OK;
+}
+
+int bpc_lut_train(const int32_t* img, int size_x, int size_y, const bpc_params* p, int32_t* host_lut, int lut_len,
+                  int nthreads) {
+  int st = check_params(p);
+  if (st) return st;
+  st = check_image(size_x, size_y, p);
+  if (st) return st;
+  if (!img || !host_lut) return fail(BPC_ERR_ARG, "null buffer");
+  if (p->layout != BPC_LAYOUT_RASTER) return fail(BPC_ERR_ARG, "training image must be raster");
+  if (lut_len < bpc_lut_len(p)) return fail(BPC_ERR_ARG, "lut_len too small");
+  const int G = 3 * p->dwt_levels + 1;
+  const int nbp = p->lut_n_bitplanes;
+  const size_t ncnt = (size_t)5 * G * nbp * 9;
+  const int ncb = (size_x / 64) * (size_y / p->cblock_length);
+  std::vector<uint32_t> n0(ncnt, 0), n(ncnt, 0);
+  int nt = nthreads > 0 ? nthreads : (int)std::thread::hardware_concurrency();
+  nt = std::max(1, std::min(nt, std::max(1, ncb)));
+  std::atomic<int> next(0);
+  std::mutex mu;
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nt; ++t)
+    pool.emplace_back([&]() {
+      std::vector<uint32_t> l0(ncnt, 0), ln(ncnt, 0);
+      for (;;) {
+        const int cb0 = next.fetch_add(8);
+        if (cb0 >= ncb) break;
+        for (int cb = cb0; cb < std::min(ncb, cb0 + 8); ++cb)
+          train_codeblock(img, size_x, size_y, cb, p, G, nbp, l0.data(), ln.data());
+      }
+      std::lock_guard<std::mutex> lk(mu);
+      for (size_t k = 0; k < ncnt; ++k) {
+        n0[k] += l0[k];
+        n[k] += ln[k];
+      }
+    });
+  for (auto& th : pool) th.join();
+  // layout: ref | sig | sign | cp_sig | cp_sign (BPC_PaCo.cu:228-250, :988-989)
+  const int nctx[5] = {1, 9, 4, 9, 4};
+  int base = 0;
+  for (int region = 0; region < (p->cpass3 ? 5 : 3); ++region) {
+    for (int g = 0; g < G; ++g)
+      for (int bp = 0; bp < nbp; ++bp)
+        for (int ctx = 0; ctx < nctx[region]; ++ctx) {
+          const size_t idx = (((size_t)region * G + g) * nbp + bp) * 9 + ctx;
+          int v = 64;
+          if (n[idx]) {
+            v = (int)(((uint64_t)n0[idx] * 128 + n[idx] / 2) / n[idx]);
+            v = std::min(127, std::max(1, v));
+          }
+          host_lut[base + (g * nbp + bp) * nctx[region] + ctx] = v;
+        }
+    base += G * nbp * nctx[region];
+  }
+  return BPC_OK;
+}
+
+int bpc_dev_malloc(void** ptr, size_t bytes) {
+  if (!ptr) return fail(BPC_ERR_ARG, "null pointer");
+  HIP_TRY(hipMalloc(ptr, bytes));
+  return BPC_OK;
+}
+int bpc_dev_free(void* ptr) {
+  if (ptr) HIP_TRY(hipFree(ptr));
+  return BPC_OK;
+}
+int bpc_dev_memset(void* ptr, int value, size_t bytes, void* stream) {
+  HIP_TRY(hipMemsetAsync(ptr, value, bytes, (hipStream_t)stream));
+  return BPC_OK;
+}
+int bpc_copy_h2d(void* dst, const void* src, size_t bytes, void* stream) {
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+  HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+  return BPC_OK;
+}
+int bpc_copy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+  return BPC_OK;
+}
+int bpc_stream_create(void** stream) {
+  hipStream_t s;
+  HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return BPC_OK;
+}
+int bpc_stream_destroy(void* stream) {
+  if (stream) HIP_TRY(hipStreamDestroy((hipStream_t)stream));
+  return BPC_OK;
+}
+int bpc_stream_sync(void* stream) {
+  HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+  return BPC_OK;
+}
+int bpc_device_sync(void) {
+  HIP_TRY(hipDeviceSynchronize());
+  return BPC_OK;
+}
+int bpc_event_create(void** ev) {
+  hipEvent_t e;
+  HIP_TRY(hipEventCreate(&e));
+  *ev = e;
+  return BPC_OK;
+}
+int bpc_event_destroy(void* ev) {
+  if (ev) HIP_TRY(hipEventDestroy((hipEvent_t)ev));
+  return BPC_OK;
+}
+int bpc_event_record(void* ev, void* stream) {
+  HIP_TRY(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream));
+  return BPC_OK;
+}
+int bpc_event_elapsed_ms(void* start, void* stop, float* ms) {
+  HIP_TRY(hipEventSynchronize((hipEvent_t)stop));
+  HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+  return BPC_OK;
+}
+int bpc_mem_info(size_t* free_bytes, size_t* total_bytes) {
+  HIP_TRY(hipMemGetInfo(free_bytes, total_bytes));
+  return BPC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,516 @@
+// BPC-PaCo codeblock encoder / decoder for gfx950 (CDNA4, wave64).
+//
+// Work mapping.  The codestream format depends on 32 arithmetic coders per
+// 64-column codeblock (one per 2-column stripe, BPC_PaCo.cu:1124-1130), so one
+// wave64 carries two codeblocks: lanes 0-31 code codeblock 2w, lanes 32-63 code
+// codeblock 2w+1.  Lane l of a half owns columns 2(l%32) and 2(l%32)+1.
+//
+// State layout.  Instead of the reference's 128 packed words per lane
+// (TData[], BPC_PaCo.cu:1120) each lane keeps its stripe bit-sliced: one 64-bit
+// row mask per column for each flag (significant S, refine R, cleanup C, sign),
+// so a context is a popcount of three 3-row windows and a flag update is one OR.
+// The encoder's magnitude bitplanes (and the decoder's decoded bitplanes) sit
+// in a register shift register of MAXP planes, so every index is static.
+//
+// Cross-lane traffic per scan step: one DPP wave shift of a packed word (the
+// neighbour column's 3-row significance window and its sign bit); codeword
+// slots: one 64-bit ballot per coder call site, ranked per 32-lane half with
+// mbcnt, and two wave-uniform (SGPR) slot counters, one per half.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bpc_internal.h"
+
+namespace bpc {
+namespace {
+
+typedef uint64_t u64;
+typedef uint32_t u32;
+
+constexpr int N_SIG = 9;   // N_CONTEXT_SIGNIFICANCE (common.h:33)
+constexpr int N_SIGN = 4;  // N_CONTEXT_SIGN (common.h:32)
+
+// Sign context of (sgn h, sgn v), BPC_PaCo.cu:178-211, as 4-bit entries at index
+// 3*hs + vs with hs, vs in {0: zero, 1: positive, 2: negative}.
+constexpr u64 SIGN_CTX_TABLE = 0x715064320ull;
+
+struct PlaneLut {  // probabilities of one bitplane, 7 bits each
+  u64 sig;         // 9 significance contexts, entry k at bits 7k
+  u64 cpsig;       // cleanup-pass significance contexts
+  u32 sign;        // 4 sign contexts, entry k at bits 8k
+  u32 cpsign;
+  u32 ref;         // refinement probability
+};
+
+template <int MAXP>
+struct LaneState {
+  u64 S[2], R[2], C[2], G[2];  // row masks per column (0 even, 1 odd): significant, refine, cleanup, sign
+  u64 P[MAXP][2];              // encoder: magnitude planes ; decoder: decoded planes
+  u64 cur[2];                  // decoder: bits decoded in the current plane
+  u32 L, Sz, slot, cw;         // coder: interval lower, interval size, reserved slot, codeword
+  u32 cnt0, cnt1;              // next free slot of each half (wave-uniform)
+};
+
+__device__ __forceinline__ u32 dpp_from_left(u32 v) {  // lane l receives lane l-1
+  return (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xF, 0xF, true);  // wave_shr:1
+}
+__device__ __forceinline__ u32 dpp_from_right(u32 v) {  // lane l receives lane l+1
+  return (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xF, 0xF, true);  // wave_shl:1
+}
+
+// Find_Subband (BPC_PaCo.cu:84-111) as a LUT group index level*3 + subband.
+__device__ __forceinline__ int find_group(int x, int y, int X, int Y, int levels) {
+  for (int aux = 1; aux <= levels; ++aux) {
+    const bool bx = x >= (X >> aux), by = y >= (Y >> aux);
+    if (bx || by) return (aux - 1) * 3 + (bx ? (by ? 2 : 0) : 1);
+  }
+  return levels * 3;
+}
+
+__device__ __forceinline__ int sgn3(int v) { return v > 0 ? 1 : (v < 0 ? 2 : 0); }
+
+__device__ __forceinline__ int chi(u32 s, u32 neg) { return s ? (neg ? -1 : 1) : 0; }
+
+struct Geo {
+  int half;        // 0: lanes 0-31, 1: lanes 32-63
+  int hl;          // lane within the half
+  int64_t region;  // first word of the codeblock's region
+  int stride;
+  int rows;
+};
+
+// Slot reservation for one coder call site (BPC_PaCo.cu:268-277 / :304-316):
+// lanes at the site whose interval is exhausted take consecutive slots in lane
+// order after the half's counter; the counter advances by their number.
+template <int DEC, int MAXP>
+__device__ __forceinline__ void reserve(LaneState<MAXP>& s, bool at, const Geo& g, const int32_t* src) {
+  const bool res = at && s.Sz == 0;
+  const u64 m = __ballot(res);
+  if (m) {
+    const u32 lo = (u32)m, hi = (u32)(m >> 32);
+    const u32 rank = __builtin_amdgcn_mbcnt_hi(g.half ? hi : 0u, __builtin_amdgcn_mbcnt_lo(g.half ? 0u : lo, 0u));
+    const u32 base = g.half ? s.cnt1 : s.cnt0;
+    if (res) {
+      s.slot = base + rank;
+      s.L = 0;
+      s.Sz = 0xFFFFu;
+      if (DEC) s.cw = s.slot < (u32)g.stride ? (u32)src[g.region + s.slot] : 0u;
+    }
+    s.cnt0 += __builtin_popcount(lo);
+    s.cnt1 += __builtin_popcount(hi);
+  }
+}
+
+// Encoder coder step (BPC_PaCo.cu:281-294).
+template <int MAXP>
+__device__ __forceinline__ void enc_symbol(LaneState<MAXP>& s, u32 sym, u32 prob, const Geo& g, int32_t* dst) {
+  const u32 aux = ((s.Sz * prob) >> 7) + sym;
+  if (sym) {
+    s.Sz -= aux;
+    s.L += aux;
+  } else {
+    s.Sz = aux;
+  }
+  if (s.Sz == 0 && s.slot < (u32)g.stride) dst[g.region + s.slot] = (int32_t)s.L;
+}
+
+// Decoder coder step (BPC_PaCo.cu:321-337).
+template <int MAXP>
+__device__ __forceinline__ u32 dec_symbol(LaneState<MAXP>& s, u32 prob) {
+  const u32 aux = ((s.Sz * prob) >> 7) + 1u;
+  const u32 aux2 = s.L + aux;
+  if (s.cw >= aux2) {
+    s.Sz -= aux;
+    s.L = aux2;
+    return 1u;
+  }
+  s.Sz = aux - 1u;
+  return 0u;
+}
+
+// One scan step of the significance pass (CLEAN = 0, SPP_Encoder/SPP_Decoder,
+// BPC_PaCo.cu:412-507) or of the cleanup pass (CLEAN = 1, CP_Encoder/CP_Decoder,
+// BPC_PaCo.cu:510-589) at row i of column ODD.
+template <int ODD, bool FIRST, int DEC, int CP3, int CLEAN, int MAXP>
+__device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u64 plane, const PlaneLut& pl,
+                                         const Geo& g, const int32_t* src, int32_t* dst) {
+  constexpr int c = ODD, o = 1 - ODD;
+  const u64 bit = 1ull << i;
+  // 3-row windows (rows i-1, i, i+1 at bits 0..2); rows -1 and past the end are 0.
+  const u32 wo = FIRST ? ((u32)(s.S[o] << 1) & 7u) : ((u32)(s.S[o] >> (i - 1)) & 7u);
+  const u32 wc = FIRST ? ((u32)(s.S[c] << 1) & 5u) : ((u32)(s.S[c] >> (i - 1)) & 5u);
+  // the neighbour lane's column (left lane's odd column for an even step, right
+  // lane's even column for an odd step; BPC_PaCo.cu:342-388) with its sign bit at row i
+  const u32 pk = wo | (((u32)(s.G[o] >> i) & 1u) << 3);
+  u32 nb = ODD ? dpp_from_right(pk) : dpp_from_left(pk);
+  if (g.hl == (ODD ? 31 : 0)) nb = 0;  // Correct_CB_borders, BPC_PaCo.cu:390-410
+  const u32 ctx = __builtin_popcount(wo) + __builtin_popcount(wc) + __builtin_popcount(nb & 7u);
+
+  bool at;
+  if (!CLEAN) {
+    const bool cand = act && !((s.S[c] >> i) & 1u);
+    if (CP3) {
+      const bool gate = ctx != 0;  // BPC_PaCo.cu:424
+      if (cand && !gate) s.C[c] |= bit;
+      at = cand && gate;
+    } else {
+      at = cand;
+    }
+  } else {
+    at = act && ((s.C[c] >> i) & 1u);  // BPC_PaCo.cu:515
+    if (at) s.C[c] &= ~bit;            // BPC_PaCo.cu:528
+  }
+  // site 1: significance symbol
+  reserve<DEC>(s, at, g, src);
+  u32 sym = 0;
+  if (at) {
+    const u32 prob = (u32)((CLEAN ? pl.cpsig : pl.sig) >> (7u * ctx)) & 127u;
+    if (!DEC) {
+      sym = (u32)(plane >> i) & 1u;
+      enc_symbol(s, sym, prob, g, dst);
+    } else {
+      sym = dec_symbol(s, prob);
+    }
+  }
+  const bool nsig = at && sym;
+  if (nsig) {
+    s.S[c] |= bit;
+    if (CLEAN) s.R[c] |= bit;  // BPC_PaCo.cu:534
+    if (DEC) s.cur[c] |= bit;  // magnitude bit i of this plane
+  }
+  // site 2: sign symbol of the coefficients that became significant
+  if (__ballot(nsig)) {
+    // own column rows i-1 (bit 0) and i+1 (bit 2): vertical neighbours U and D
+    const u32 gc = FIRST ? ((u32)(s.G[c] << 1) & 5u) : ((u32)(s.G[c] >> (i - 1)) & 5u);
+    const int chi_u = chi(wc & 1u, gc & 1u);
+    const int chi_d = chi((wc >> 2) & 1u, (gc >> 2) & 1u);
+    const int chi_n = chi((nb >> 1) & 1u, (nb >> 3) & 1u);
+    const int chi_o = chi((u32)(s.S[o] >> i) & 1u, (u32)(s.G[o] >> i) & 1u);
+    const int k = 3 * sgn3(chi_n + chi_o) + sgn3(chi_u + chi_d);
+    const u32 sctx = (u32)(SIGN_CTX_TABLE >> (4 * k)) & 0xFu;
+    reserve<DEC>(s, nsig, g, src);
+    if (nsig) {
+      const u32 prob = ((CLEAN ? pl.cpsign : pl.sign) >> (8u * (sctx >> 1))) & 127u;
+      if (!DEC) {
+        enc_symbol(s, ((u32)(s.G[c] >> i) & 1u) ^ (sctx & 1u), prob, g, dst);
+      } else {
+        const u32 v = dec_symbol(s, prob) ^ (sctx & 1u);
+        if (v) s.G[c] |= bit;
+      }
+    }
+  }
+}
+
+// One scan step of the refinement pass (MRP_Encoder/MRP_Decoder, BPC_PaCo.cu:593-629).
+template <int ODD, int DEC, int MAXP>
+__device__ __forceinline__ void ref_step(LaneState<MAXP>& s, int i, bool act, u64 plane, const PlaneLut& pl,
+                                         const Geo& g, const int32_t* src, int32_t* dst) {
+  constexpr int c = ODD;
+  const u64 bit = 1ull << i;
+  const bool r = (s.R[c] >> i) & 1u;
+  const bool at = act && r;
+  if (act && !r && ((s.S[c] >> i) & 1u)) s.R[c] |= bit;
+  reserve<DEC>(s, at, g, src);
+  if (at) {
+    if (!DEC) {
+      enc_symbol(s, (u32)(plane >> i) & 1u, pl.ref, g, dst);
+    } else {
+      if (dec_symbol(s, pl.ref)) s.cur[c] |= bit;
+    }
+  }
+}
+
+template <int DEC, int CP3, int CLEAN, int MAXP>
+__device__ __forceinline__ void sig_pass(LaneState<MAXP>& s, bool act, const u64* plane, const PlaneLut& pl,
+                                         const Geo& g, const int32_t* src, int32_t* dst) {
+  sig_step<0, true, DEC, CP3, CLEAN>(s, 0, act, plane[0], pl, g, src, dst);
+  sig_step<1, true, DEC, CP3, CLEAN>(s, 0, act, plane[1], pl, g, src, dst);
+  for (int i = 1; i < g.rows; ++i) {
+    sig_step<0, false, DEC, CP3, CLEAN>(s, i, act, plane[0], pl, g, src, dst);
+    sig_step<1, false, DEC, CP3, CLEAN>(s, i, act, plane[1], pl, g, src, dst);
+  }
+}
+
+template <int DEC, int MAXP>
+__device__ __forceinline__ void ref_pass(LaneState<MAXP>& s, bool act, const u64* plane, const PlaneLut& pl,
+                                         const Geo& g, const int32_t* src, int32_t* dst) {
+  for (int i = 0; i < g.rows; ++i) {
+    ref_step<0, DEC>(s, i, act, plane[0], pl, g, src, dst);
+    ref_step<1, DEC>(s, i, act, plane[1], pl, g, src, dst);
+  }
+}
+
+__device__ __forceinline__ u32 lut_at(const int32_t* lut, int idx, bool& bad) {
+  const int32_t v = lut[idx];
+  bad |= (u32)v > 127u;
+  return (u32)v & 127u;
+}
+
+template <int CP3>
+__device__ __forceinline__ PlaneLut load_plane_lut(const int32_t* lut, int ref0, int sig0, int sign0, int cpoff,
+                                                   int bp, bool& bad) {
+  PlaneLut pl;
+  pl.sig = 0;
+  pl.cpsig = 0;
+  pl.sign = 0;
+  pl.cpsign = 0;
+#pragma unroll
+  for (int k = 0; k < N_SIG; ++k) pl.sig |= (u64)lut_at(lut, sig0 + bp * N_SIG + k, bad) << (7 * k);
+#pragma unroll
+  for (int k = 0; k < N_SIGN; ++k) pl.sign |= lut_at(lut, sign0 + bp * N_SIGN + k, bad) << (8 * k);
+  pl.ref = lut_at(lut, ref0 + bp, bad);
+  if (CP3) {
+#pragma unroll
+    for (int k = 0; k < N_SIG; ++k) pl.cpsig |= (u64)lut_at(lut, cpoff + sig0 + bp * N_SIG + k, bad) << (7 * k);
+#pragma unroll
+    for (int k = 0; k < N_SIGN; ++k) pl.cpsign |= lut_at(lut, cpoff + sign0 + bp * N_SIGN + k, bad) << (8 * k);
+  }
+  return pl;
+}
+
+template <int DEC, int CP3, int MAXP>
+__global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int half = lane >> 5;
+  const int hl = lane & 31;
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int cb = 2 * wave + half;
+  const bool valid = cb < a.ncb;
+  const int cbc = valid ? cb : a.ncb - 1;
+  const int tx = cbc % a.nx, ty = cbc / a.nx;
+  const int x = tx * 64 + 2 * hl;  // TCoordinate_X, BPC_PaCo.cu:1126
+  const int y0 = ty * a.rows;      // TCoordinate_Y, BPC_PaCo.cu:1127
+
+  Geo g;
+  g.half = half;
+  g.hl = hl;
+  g.region = (int64_t)cbc * a.stride;  // WCoordinate, BPC_PaCo.cu:1130
+  g.stride = a.stride;
+  g.rows = a.rows;
+  int64_t cbase;
+  int rs;
+  if (a.layout) {
+    cbase = (int64_t)cbc * 64 * a.rows + 2 * hl;
+    rs = 64;
+  } else {
+    cbase = (int64_t)y0 * a.size_x + x;
+    rs = a.size_x;
+  }
+
+  LaneState<MAXP> s;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    s.S[c] = s.R[c] = s.C[c] = s.G[c] = s.cur[c] = 0;
+#pragma unroll
+    for (int p = 0; p < MAXP; ++p) s.P[p][c] = 0;
+  }
+  s.L = 0;
+  s.Sz = 0;
+  s.slot = 0;
+  s.cw = 0;
+  s.cnt0 = 1;
+  s.cnt1 = 1;
+  if (CP3) {  // every coefficient starts with the cleanup flag (BPC_PaCo.cu:34-39, 73-78)
+    const u64 rm = a.rows >= 64 ? ~0ull : ((1ull << a.rows) - 1ull);
+    s.C[0] = s.C[1] = rm;
+  }
+
+  u32 status = 0;
+  int msb;
+  bool good = valid;
+  if (!DEC) {
+    // Read_Coefficients (BPC_PaCo.cu:14-44) straight into bit-sliced planes.
+    u32 mor = 0;
+    for (int h = 0; h < 2; ++h) {
+      const int r0 = 32 * h, r1 = min(a.rows, 32 * h + 32);
+      u32 plo[MAXP][2];
+#pragma unroll
+      for (int p = 0; p < MAXP; ++p) plo[p][0] = plo[p][1] = 0;
+      u32 glo0 = 0, glo1 = 0;
+#pragma unroll 4
+      for (int r = r0; r < r1; ++r) {
+        int2 v = make_int2(0, 0);
+        if (valid) v = *reinterpret_cast<const int2*>(a.src + cbase + (int64_t)r * rs);
+        const u32 m0 = v.x < 0 ? 0u - (u32)v.x : (u32)v.x;
+        const u32 m1 = v.y < 0 ? 0u - (u32)v.y : (u32)v.y;
+        mor |= m0 | m1;
+        const int sh = r - r0;
+        glo0 |= (u32)(v.x < 0) << sh;
+        glo1 |= (u32)(v.y < 0) << sh;
+#pragma unroll
+        for (int p = 0; p < MAXP; ++p) {
+          plo[p][0] |= ((m0 >> p) & 1u) << sh;
+          plo[p][1] |= ((m1 >> p) & 1u) << sh;
+        }
+      }
+      s.G[0] |= (u64)glo0 << (32 * h);
+      s.G[1] |= (u64)glo1 << (32 * h);
+#pragma unroll
+      for (int p = 0; p < MAXP; ++p) {
+        s.P[p][0] |= (u64)plo[p][0] << (32 * h);
+        s.P[p][1] |= (u64)plo[p][1] << (32 * h);
+      }
+    }
+    // Find_MSB (BPC_PaCo.cu:114-161): OR over the 32 lanes of the half.
+#pragma unroll
+    for (int off = 1; off < 32; off <<= 1) mor |= (u32)__shfl_xor((int)mor, off, 64);
+    msb = mor ? 31 - __clz(mor) : 32;
+    if (valid && hl == 0) a.dst[g.region] = msb;  // BPC_PaCo.cu:1155
+    if (msb != 32 && msb > 27) {
+      status |= ST_RANGE;
+      good = false;
+    }
+  } else {
+    msb = valid ? a.src[g.region] : 32;  // BPC_PaCo.cu:1216
+    if (msb != 32 && (msb < 0 || msb > 27)) {
+      status |= ST_MALFORMED;
+      good = false;
+    }
+  }
+  if (msb == 32) good = false;  // all-zero codeblock: nothing coded
+  if (good && msb >= MAXP) {
+    status |= ST_WIDE;
+    good = false;
+  }
+
+  // LUT row pointers (Initialize_LUT_pointers, BPC_PaCo.cu:228-250), plane 0 rows.
+  const int G = 3 * a.levels + 1;
+  const int grp = find_group(x, y0, a.size_x, a.size_y, a.levels);
+  const int ref0 = grp * a.nbp;
+  const int sig0 = G * a.nbp + grp * a.nbp * N_SIG;
+  const int sign0 = G * a.nbp * (1 + N_SIG) + grp * a.nbp * N_SIGN;
+  const int cpoff = G * a.nbp * (N_SIG + N_SIGN);  // BPC_PaCo.cu:988-989
+  if (good) {
+    const int cp = CP3 ? cpoff : 0;
+    if (ref0 + msb + 1 > a.lut_len || sig0 + cp + msb * N_SIG + N_SIG > a.lut_len ||
+        sign0 + cp + msb * N_SIGN + N_SIGN > a.lut_len) {
+      status |= ST_RANGE;
+      good = false;
+    }
+  }
+  // a codeblock is coded only if every lane of its half is good
+  {
+    const u64 ok = __ballot(good);
+    const u64 hm = half ? 0xFFFFFFFF00000000ull : 0x00000000FFFFFFFFull;
+    const u64 vm = __ballot(valid);
+    good = good && ((ok & hm) == (vm & hm));
+  }
+  const int msb_eff = good ? msb : -1;
+  const int top = max(__builtin_amdgcn_readlane(msb_eff, 0), __builtin_amdgcn_readlane(msb_eff, 32));
+
+  if (!DEC) {
+    // align the shift register so that P[MAXP-1] holds plane `top`
+    for (int k = top; k < MAXP - 1; ++k) {
+#pragma unroll
+      for (int p = MAXP - 1; p > 0; --p) {
+        s.P[p][0] = s.P[p - 1][0];
+        s.P[p][1] = s.P[p - 1][1];
+      }
+      s.P[0][0] = s.P[0][1] = 0;
+    }
+  }
+
+  // Bitplane driver (Encode / Decode, BPC_PaCo.cu:968-1090).  In 3-pass mode the
+  // reference's opening cleanup pass at MSB equals plane MSB with SPP and MRP
+  // skipped (nothing is significant yet, so they code nothing).
+  for (int bp = top; bp >= 0; --bp) {
+    const bool act = good && bp <= msb;
+    bool lbad = false;
+    PlaneLut pl;
+    pl.sig = pl.cpsig = 0;
+    pl.sign = pl.cpsign = pl.ref = 0;
+    if (act) pl = load_plane_lut<CP3>(a.lut, ref0, sig0, sign0, cpoff, bp, lbad);
+    if (lbad) status |= ST_RANGE;
+    u64 plane[2] = {0, 0};
+    if (!DEC) {
+      plane[0] = s.P[MAXP - 1][0];
+      plane[1] = s.P[MAXP - 1][1];
+    }
+    const bool act_sr = act && !(CP3 && bp == msb);
+    if (__ballot(act_sr)) {
+      sig_pass<DEC, CP3, 0>(s, act_sr, plane, pl, g, a.src, a.dst);
+      ref_pass<DEC>(s, act_sr, plane, pl, g, a.src, a.dst);
+    }
+    if (CP3) sig_pass<DEC, CP3, 1>(s, act, plane, pl, g, a.src, a.dst);
+    if (!DEC) {
+#pragma unroll
+      for (int p = MAXP - 1; p > 0; --p) {
+        s.P[p][0] = s.P[p - 1][0];
+        s.P[p][1] = s.P[p - 1][1];
+      }
+      s.P[0][0] = s.P[0][1] = 0;
+    } else {
+#pragma unroll
+      for (int p = 0; p < MAXP - 1; ++p) {
+        s.P[p][0] = s.P[p + 1][0];
+        s.P[p][1] = s.P[p + 1][1];
+      }
+      s.P[MAXP - 1][0] = s.cur[0];
+      s.P[MAXP - 1][1] = s.cur[1];
+      s.cur[0] = s.cur[1] = 0;
+    }
+  }
+
+  const u32 cnt = half ? s.cnt1 : s.cnt0;  // next free slot of this codeblock
+  const bool overflow = good && cnt > (u32)a.stride;
+  if (overflow) status |= ST_OVERFLOW;
+  if (!DEC) {
+    // flush of each lane's open codeword (BPC_PaCo.cu:1020)
+    if (good && s.slot < (u32)a.stride) a.dst[g.region + s.slot] = (int32_t)s.L;
+    if (good && a.fill)
+      for (u32 j = cnt + hl; j < (u32)a.stride; j += 32) a.dst[g.region + j] = -1;
+    if (valid && hl == 0 && a.ncw) a.ncw[cb] = good ? (int32_t)(cnt - 1) : 0;
+  } else {
+    // Write_Coefficients (BPC_PaCo.cu:46-63): magnitude from the decoded planes
+    const bool emit = good && !overflow;
+    if (valid) {
+      for (int r = 0; r < a.rows; ++r) {
+        u32 m0 = 0, m1 = 0;
+#pragma unroll
+        for (int p = 0; p < MAXP; ++p) {
+          m0 |= ((u32)(s.P[MAXP - 1 - p][0] >> r) & 1u) << p;
+          m1 |= ((u32)(s.P[MAXP - 1 - p][1] >> r) & 1u) << p;
+        }
+        int v0 = (int)m0, v1 = (int)m1;
+        if ((s.G[0] >> r) & 1u) v0 = -v0;
+        if ((s.G[1] >> r) & 1u) v1 = -v1;
+        if (!emit) v0 = v1 = 0;
+        *reinterpret_cast<int2*>(a.dst + cbase + (int64_t)r * rs) = make_int2(v0, v1);
+      }
+    }
+  }
+  if (status) atomicOr(a.status, status);
+}
+
+}  // namespace
+
+int planes_for(int nbp) {
+  if (nbp <= 16) return 16;
+  if (nbp <= 20) return 20;
+  return 28;
+}
+
+template <int DEC, int CP3, int MAXP>
+static hipError_t launch_one(const KArgs& a, hipStream_t stream) {
+  const int waves = (a.ncb + 1) / 2;
+  hipLaunchKernelGGL((bpc_codec_kernel<DEC, CP3, MAXP>), dim3(waves), dim3(64), 0, stream, a);
+  return hipGetLastError();
+}
+
+template <int DEC, int CP3>
+static hipError_t launch_p(int maxp, const KArgs& a, hipStream_t stream) {
+  switch (maxp) {
+    case 16: return launch_one<DEC, CP3, 16>(a, stream);
+    case 20: return launch_one<DEC, CP3, 20>(a, stream);
+    case 28: return launch_one<DEC, CP3, 28>(a, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_codec(int decode, int cpass3, int maxp, const KArgs& a, hipStream_t stream) {
+  if (a.ncb <= 0) return hipSuccess;
+  if (decode) return cpass3 ? launch_p<1, 1>(maxp, a, stream) : launch_p<1, 0>(maxp, a, stream);
+  return cpass3 ? launch_p<0, 1>(maxp, a, stream) : launch_p<0, 0>(maxp, a, stream);
+}
+
+}  // namespace bpc

@@ -1,0 +1,132 @@
+/*
+ * bpc_paco.h -- C-ABI of the MI355X BPC-PaCo codec (libbpc_paco.so).
+ *
+ * Drop-in boundary for the reference's host entry point
+ *     void CUDA_BPC(int Direction, DATATYPE* HData, int HDSize_X, int HDSize_Y, int* LUT)
+ * (source/BPC_PaCo.cu:1305-1325, declared for callers at source/main_example.cu:8)
+ * and for the LUT loader load_LUTs (source/main_example.cu:219-448).
+ *
+ * Plain pointers and sizes only.  Every function returns BPC_OK (0) or a negative
+ * status; bpc_last_error() gives a message for the calling thread.
+ */
+#ifndef BPC_PACO_H
+#define BPC_PACO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference prints and exit()s instead, common.cu:3-8) ---- */
+#define BPC_OK 0
+#define BPC_ERR_ARG (-1)       /* invalid argument or configuration */
+#define BPC_ERR_HIP (-2)       /* HIP runtime error (allocation, copy, launch) */
+#define BPC_ERR_OVERFLOW (-3)  /* a codeblock needs more codeword slots than its region holds */
+#define BPC_ERR_RANGE (-4)     /* |coefficient| >= 2^28, LUT entry outside [0,127], or LUT row outside the array */
+#define BPC_ERR_MALFORMED (-5) /* decoder: MSB word of a region not in [0,27] and not 32 */
+#define BPC_ERR_WIDE (-6)      /* device-resident call: a codeblock's MSB is beyond the kernel's bitplane
+                                  registers (only when MSB >= lut_n_bitplanes); bpc_run_ex retries itself */
+#define BPC_ERR_NODEVICE (-7)  /* no HIP device */
+
+/* ---- direction (enum BPC, common.h:8-11) ---- */
+#define BPC_CODE 0
+#define BPC_DECODE 1
+
+/* ---- coefficient layouts ---- */
+#define BPC_LAYOUT_RASTER 0 /* row-major X*Y image (the reference's layout) */
+#define BPC_LAYOUT_TILED 1  /* codeblock-contiguous: codeblock t (row-major tile order) at t*64*rows,
+                               row-major inside the codeblock */
+
+/* Runtime form of the reference's compile-time configuration (common.h:19-51). */
+typedef struct bpc_params {
+  int32_t cpass3;          /* CPASS_3 (common.h:24): 0 = SPP+MRP, 1 = SPP+MRP+CP. Reference default 0 */
+  int32_t dwt_levels;      /* DWT_LEVELS (common.h:19). Default 5 */
+  int32_t cblock_length;   /* CBLOCK_LENGTH (common.h:45): codeblock rows, 2..64. Default 64 */
+  int32_t lut_n_bitplanes; /* LUT_N_BITPLANES (common.h:28): LUT rows per group. Default 15 */
+  int32_t cb_stride_words; /* int32 words per codeblock region; 0 = 64*cblock_length, the reference's
+                              layout (BPC_PaCo.cu:1130), in which the codestream has the image's size */
+  int32_t layout;          /* BPC_LAYOUT_RASTER (default) or BPC_LAYOUT_TILED */
+} bpc_params;
+
+/* Reference defaults: CPASS_3=0, DWT_LEVELS=5, CBLOCK_LENGTH=64, LUT_N_BITPLANES=15. */
+void bpc_default_params(bpc_params* p);
+/* LUT length in ints: (3*L+1)*NBP*(1+9+4), plus (3*L+1)*NBP*(9+4) when cpass3
+ * (refinement_N + significance_N + sign_N [+ significance_N + sign_N], main_example.cu:266-278). */
+int bpc_lut_len(const bpc_params* p);
+/* int32 words of one codestream buffer for an image: codeblocks * stride. */
+int64_t bpc_stream_words(int size_x, int size_y, const bpc_params* p);
+const char* bpc_last_error(void);
+int bpc_device_count(void);
+
+/* ---- drop-in entry point ------------------------------------------------------
+ * Replaces CUDA_BPC (BPC_PaCo.cu:1305-1325).  hdata is a host buffer of
+ * size_x*size_y int32, overwritten in place: CODE turns two's-complement
+ * coefficients into per-codeblock codestream regions (word 0 = MSB, then the
+ * codewords, unused words = -1); DECODE turns the regions back into coefficients.
+ * d_lut is a device LUT (from bpc_lut_load_text / bpc_lut_upload).
+ * Deviation: the reference frees d_lut (BPC_PaCo.cu:1300); this library leaves it
+ * to the caller (bpc_lut_free).  Uses the configuration set by bpc_set_config
+ * (reference defaults until then). */
+int bpc_run(int direction, int32_t* hdata, int size_x, int size_y, const int32_t* d_lut);
+int bpc_set_config(const bpc_params* p);
+int bpc_get_config(bpc_params* p);
+/* Same with explicit configuration and LUT length.  With a non-default
+ * cb_stride_words the codestream is bpc_stream_words() words: hdata must hold
+ * max(size_x*size_y, bpc_stream_words()) int32. */
+int bpc_run_ex(int direction, int32_t* hdata, int size_x, int size_y, const int32_t* d_lut, int lut_len,
+               const bpc_params* p);
+
+/* ---- probability LUTs (main_example.cu:219-448) ---- */
+/* Loads ref.txt, sig.txt, sign.txt (+ cp_sig.txt, cp_sign.txt when cpass3) from
+ * folder with the reference loader's layout and padding rule and uploads them.
+ * *nbp receives max_bitplane+1 (the LUT_N_BITPLANES the table was laid out for). */
+int bpc_lut_load_text(const char* folder, int cpass3, int dwt_levels, int32_t** d_lut, int* lut_len, int* nbp);
+/* Same, into a host array of capacity cap ints (no device needed). */
+int bpc_lut_parse_text(const char* folder, int cpass3, int dwt_levels, int32_t* host_lut, int cap, int* lut_len,
+                       int* nbp);
+int bpc_lut_upload(const int32_t* host_lut, int lut_len, int32_t** d_lut);
+int bpc_lut_free(int32_t* d_lut);
+/* Builds a LUT by counting symbol-0 frequencies of every (pass, level, subband,
+ * bitplane, context) over the codeblocks of a training image (raster, two's
+ * complement), with the coder's own context modelling.  P = clamp(round(128*n0/n), 1, 127),
+ * 64 where a row was never visited.  nthreads <= 0: all host threads. */
+int bpc_lut_train(const int32_t* img, int size_x, int size_y, const bpc_params* p, int32_t* host_lut, int lut_len,
+                  int nthreads);
+
+/* ---- device-resident codec (no host copies; asynchronous on `stream`) ----------
+ * d_coeffs: size_x*size_y int32 in p->layout; d_stream: bpc_stream_words() int32.
+ * d_ncw (nullable): codeword slots used per codeblock (encoder).  d_status: a device
+ * uint32 that collects error bits (nullable: a library-owned word); read and clear
+ * it with bpc_status_take(), which synchronizes `stream`.  The encoder writes
+ * every word of d_stream when fill_unused != 0 (reference layout), otherwise only
+ * word 0 and the used codeword slots of each region. */
+int bpc_encode_device(const int32_t* d_coeffs, int32_t* d_stream, int size_x, int size_y, const int32_t* d_lut,
+                      int lut_len, const bpc_params* p, int fill_unused, int32_t* d_ncw, uint32_t* d_status,
+                      void* stream);
+int bpc_decode_device(const int32_t* d_stream, int32_t* d_coeffs, int size_x, int size_y, const int32_t* d_lut,
+                      int lut_len, const bpc_params* p, uint32_t* d_status, void* stream);
+/* Synchronizes stream, maps the status bits to a BPC_* code and clears them. */
+int bpc_status_take(uint32_t* d_status, void* stream);
+
+/* ---- device memory / stream / event helpers (thin HIP wrappers) ---- */
+int bpc_dev_malloc(void** ptr, size_t bytes);
+int bpc_dev_free(void* ptr);
+int bpc_dev_memset(void* ptr, int value, size_t bytes, void* stream);
+int bpc_copy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+int bpc_copy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+int bpc_stream_create(void** stream);
+int bpc_stream_destroy(void* stream);
+int bpc_stream_sync(void* stream);
+int bpc_device_sync(void);
+int bpc_event_create(void** ev);
+int bpc_event_destroy(void* ev);
+int bpc_event_record(void* ev, void* stream);
+int bpc_event_elapsed_ms(void* start, void* stop, float* ms);
+int bpc_mem_info(size_t* free_bytes, size_t* total_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,181 @@
+"""ctypes binding of the CPU oracle (oracle/_build/libbpc_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, where it is the checker.  The product path
+(cuda_bpc-paco_amd/) never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libbpc_oracle.so")
+_lib = None
+
+OK = 0
+ERR_ARG = -1
+ERR_OVERFLOW = -3
+ERR_RANGE = -4
+ERR_MALFORMED = -5
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("cpass3", ctypes.c_int32),
+        ("dwt_levels", ctypes.c_int32),
+        ("cblock_length", ctypes.c_int32),
+        ("lut_n_bitplanes", ctypes.c_int32),
+        ("cb_stride_words", ctypes.c_int32),
+    ]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError("oracle library not built: run `make -C oracle`")
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.c_void_p
+        i32 = ctypes.c_int
+        PP = ctypes.POINTER(Params)
+        L.orc_lut_len.argtypes = [PP]
+        L.orc_encode.argtypes = [P, i32, i32, P, i32, PP, P, P, i32]
+        L.orc_decode.argtypes = [P, i32, i32, P, i32, PP, P, i32]
+        L.orc_encode_list.argtypes = [P, i32, i32, P, i32, P, i32, PP, P, P, i32]
+        L.orc_decode_list.argtypes = [P, i32, i32, P, i32, P, i32, PP, P, i32]
+        L.orc_encode_cb.argtypes = [P, i32, i32, i32, P, i32, PP, P, P]
+        L.orc_decode_cb.argtypes = [P, i32, i32, i32, P, i32, PP, P]
+        L.orc_sig_context.argtypes = [P]
+        L.orc_sign_context_hv.argtypes = [i32, i32]
+        L.orc_sign_context.argtypes = [ctypes.c_uint32] * 4
+        L.orc_find_subband.argtypes = [i32, i32, i32, i32, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+        L.orc_codeblock_msb.argtypes = [P, i32, i32, i32, i32]
+        L.orc_ac_encode_seq.argtypes = [P, P, i32, P, i32]
+        L.orc_ac_decode_seq.argtypes = [P, i32, P, i32, P]
+        L.orc_lut_load_text.argtypes = [ctypes.c_char_p, i32, i32, P, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+        _lib = L
+    return _lib
+
+
+def params(cpass3=0, dwt_levels=5, cblock_length=64, lut_n_bitplanes=15, cb_stride_words=0):
+    return Params(cpass3, dwt_levels, cblock_length, lut_n_bitplanes, cb_stride_words)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def lut_len(p):
+    return lib().orc_lut_len(ctypes.byref(p))
+
+
+def stride(p):
+    return p.cb_stride_words if p.cb_stride_words > 0 else 64 * p.cblock_length
+
+
+def n_codeblocks(size_x, size_y, p):
+    return (size_x // 64) * (size_y // p.cblock_length)
+
+
+def encode(img, lut, p, nthreads=0):
+    """Encode a (Y, X) int32 image.  Returns (status, stream[nCB*stride], ncw[nCB])."""
+    img = np.ascontiguousarray(img, dtype=np.int32)
+    lut = np.ascontiguousarray(lut, dtype=np.int32)
+    Y, X = img.shape
+    n = n_codeblocks(X, Y, p)
+    stream = np.empty(n * stride(p), dtype=np.int32)
+    ncw = np.zeros(n, dtype=np.int32)
+    st = lib().orc_encode(_ptr(img), X, Y, _ptr(lut), lut.size, ctypes.byref(p), _ptr(stream), _ptr(ncw), nthreads)
+    return st, stream, ncw
+
+
+def decode(stream, size_x, size_y, lut, p, nthreads=0):
+    stream = np.ascontiguousarray(stream, dtype=np.int32)
+    lut = np.ascontiguousarray(lut, dtype=np.int32)
+    img = np.zeros((size_y, size_x), dtype=np.int32)
+    st = lib().orc_decode(_ptr(stream), size_x, size_y, _ptr(lut), lut.size, ctypes.byref(p), _ptr(img), nthreads)
+    return st, img
+
+
+def encode_list(img, cbs, lut, p, nthreads=0):
+    """Encode only the codeblocks in cbs; untouched regions stay -1."""
+    img = np.ascontiguousarray(img, dtype=np.int32)
+    lut = np.ascontiguousarray(lut, dtype=np.int32)
+    cbs = np.ascontiguousarray(cbs, dtype=np.int32)
+    Y, X = img.shape
+    n = n_codeblocks(X, Y, p)
+    stream = np.full(n * stride(p), -1, dtype=np.int32)
+    ncw = np.zeros(n, dtype=np.int32)
+    st = lib().orc_encode_list(_ptr(img), X, Y, _ptr(cbs), cbs.size, _ptr(lut), lut.size, ctypes.byref(p),
+                               _ptr(stream), _ptr(ncw), nthreads)
+    return st, stream, ncw
+
+
+def decode_list(stream, size_x, size_y, cbs, lut, p, nthreads=0):
+    stream = np.ascontiguousarray(stream, dtype=np.int32)
+    lut = np.ascontiguousarray(lut, dtype=np.int32)
+    cbs = np.ascontiguousarray(cbs, dtype=np.int32)
+    img = np.zeros((size_y, size_x), dtype=np.int32)
+    st = lib().orc_decode_list(_ptr(stream), size_x, size_y, _ptr(cbs), cbs.size, _ptr(lut), lut.size,
+                               ctypes.byref(p), _ptr(img), nthreads)
+    return st, img
+
+
+def sig_context(nb):
+    a = np.ascontiguousarray(nb, dtype=np.uint32)
+    return lib().orc_sig_context(_ptr(a))
+
+
+def sign_context_hv(h, v):
+    return lib().orc_sign_context_hv(h, v)
+
+
+def sign_context(u, l, r, d):
+    return lib().orc_sign_context(u, l, r, d)
+
+
+def find_subband(x, y, size_x, size_y, levels):
+    lv, sb = ctypes.c_int(), ctypes.c_int()
+    lib().orc_find_subband(x, y, size_x, size_y, levels, ctypes.byref(lv), ctypes.byref(sb))
+    return lv.value, sb.value
+
+
+def codeblock_msb(img, cb, cblock_length=64):
+    img = np.ascontiguousarray(img, dtype=np.int32)
+    Y, X = img.shape
+    return lib().orc_codeblock_msb(_ptr(img), X, Y, cb, cblock_length)
+
+
+def ac_encode_seq(syms, probs):
+    syms = np.ascontiguousarray(syms, dtype=np.uint8)
+    probs = np.ascontiguousarray(probs, dtype=np.int32)
+    cw = np.zeros(max(1, syms.size + 1), dtype=np.uint32)
+    n = lib().orc_ac_encode_seq(_ptr(syms), _ptr(probs), syms.size, _ptr(cw), cw.size)
+    return cw[:n].copy()
+
+
+def ac_decode_seq(cw, probs):
+    cw = np.ascontiguousarray(cw, dtype=np.uint32)
+    probs = np.ascontiguousarray(probs, dtype=np.int32)
+    out = np.zeros(probs.size, dtype=np.uint8)
+    lib().orc_ac_decode_seq(_ptr(cw), cw.size, _ptr(probs), probs.size, _ptr(out))
+    return out
+
+
+def lut_load_text(folder, cpass3, dwt_levels=5):
+    """Returns (lut int32 array, nbp)."""
+    if not folder.endswith("/"):
+        folder += "/"
+    cap = 1 << 16
+    out = np.zeros(cap, dtype=np.int32)
+    nbp, ln = ctypes.c_int(), ctypes.c_int()
+    st = lib().orc_lut_load_text(folder.encode(), cpass3, dwt_levels, _ptr(out), cap, ctypes.byref(nbp), ctypes.byref(ln))
+    if st:
+        raise RuntimeError("orc_lut_load_text failed: %d" % st)
+    return out[: ln.value].copy(), nbp.value
