@@ -34,6 +34,22 @@ constexpr int N_SIGN = 4;  // N_CONTEXT_SIGN (common.h:32)
 // 3*hs + vs with hs, vs in {0: zero, 1: positive, 2: negative}.
 constexpr u64 SIGN_CTX_TABLE = 0x715064320ull;
 
+// Sign code {0: zero, 1: positive, 2: negative} of chi(a) + chi(b) for a pair of
+// neighbours, chi = 0 when not significant, -1 when negative, +1 otherwise
+// (BPC_PaCo.cu:220-225).  Index bits: 0 = a significant, 1 = a negative,
+// 2 = b significant, 3 = b negative (negative bits only with the significant bit).
+constexpr u32 pair_sign_table() {
+  u32 t = 0;
+  for (u32 k = 0; k < 16; ++k) {
+    const int ca = (k & 1u) ? ((k & 2u) ? -1 : 1) : 0;
+    const int cb = (k & 4u) ? ((k & 8u) ? -1 : 1) : 0;
+    const int sum = ca + cb;
+    t |= (u32)(sum > 0 ? 1 : (sum < 0 ? 2 : 0)) << (2 * k);
+  }
+  return t;
+}
+constexpr u32 PAIR_SIGN_TABLE = pair_sign_table();
+
 struct PlaneLut {  // probabilities of one bitplane, 7 bits each
   u64 sig;         // 9 significance contexts, entry k at bits 7k
   u64 cpsig;       // cleanup-pass significance contexts
@@ -67,10 +83,6 @@ __device__ __forceinline__ int find_group(int x, int y, int X, int Y, int levels
   return levels * 3;
 }
 
-__device__ __forceinline__ int sgn3(int v) { return v > 0 ? 1 : (v < 0 ? 2 : 0); }
-
-__device__ __forceinline__ int chi(u32 s, u32 neg) { return s ? (neg ? -1 : 1) : 0; }
-
 struct Geo {
   int half;        // 0: lanes 0-31, 1: lanes 32-63
   int hl;          // lane within the half
@@ -81,7 +93,8 @@ struct Geo {
 
 // Slot reservation for one coder call site (BPC_PaCo.cu:268-277 / :304-316):
 // lanes at the site whose interval is exhausted take consecutive slots in lane
-// order after the half's counter; the counter advances by their number.
+// order after the half's counter; the counter advances by their number.  The
+// counters stay wave-uniform (SGPRs).
 template <int DEC, int MAXP>
 __device__ __forceinline__ void reserve(LaneState<MAXP>& s, bool at, const Geo& g, const int32_t* src) {
   const bool res = at && s.Sz == 0;
@@ -89,22 +102,20 @@ __device__ __forceinline__ void reserve(LaneState<MAXP>& s, bool at, const Geo& 
   if (m) {
     const u32 lo = (u32)m, hi = (u32)(m >> 32);
     const u32 rank = __builtin_amdgcn_mbcnt_hi(g.half ? hi : 0u, __builtin_amdgcn_mbcnt_lo(g.half ? 0u : lo, 0u));
-    const u32 base = g.half ? s.cnt1 : s.cnt0;
-    if (res) {
-      s.slot = base + rank;
-      s.L = 0;
-      s.Sz = 0xFFFFu;
-      if (DEC) s.cw = s.slot < (u32)g.stride ? (u32)src[g.region + s.slot] : 0u;
-    }
-    s.cnt0 += __builtin_popcount(lo);
-    s.cnt1 += __builtin_popcount(hi);
+    const u32 slot = (g.half ? s.cnt1 : s.cnt0) + rank;
+    s.slot = res ? slot : s.slot;
+    s.L = res ? 0u : s.L;
+    s.Sz = res ? 0xFFFFu : s.Sz;
+    if (DEC && res) s.cw = slot < (u32)g.stride ? (u32)src[g.region + slot] : 0u;
+    s.cnt0 = __builtin_amdgcn_readfirstlane(s.cnt0 + __builtin_popcount(lo));
+    s.cnt1 = __builtin_amdgcn_readfirstlane(s.cnt1 + __builtin_popcount(hi));
   }
 }
 
 // Encoder coder step (BPC_PaCo.cu:281-294).
 template <int MAXP>
 __device__ __forceinline__ void enc_symbol(LaneState<MAXP>& s, u32 sym, u32 prob, const Geo& g, int32_t* dst) {
-  const u32 aux = ((s.Sz * prob) >> 7) + sym;
+  const u32 aux = (__umul24(s.Sz, prob) >> 7) + sym;  // Sz < 2^16, prob < 2^7
   if (sym) {
     s.Sz -= aux;
     s.L += aux;
@@ -117,7 +128,7 @@ __device__ __forceinline__ void enc_symbol(LaneState<MAXP>& s, u32 sym, u32 prob
 // Decoder coder step (BPC_PaCo.cu:321-337).
 template <int MAXP>
 __device__ __forceinline__ u32 dec_symbol(LaneState<MAXP>& s, u32 prob) {
-  const u32 aux = ((s.Sz * prob) >> 7) + 1u;
+  const u32 aux = (__umul24(s.Sz, prob) >> 7) + 1u;
   const u32 aux2 = s.L + aux;
   if (s.cw >= aux2) {
     s.Sz -= aux;
@@ -180,14 +191,16 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
   }
   // site 2: sign symbol of the coefficients that became significant
   if (__ballot(nsig)) {
-    // own column rows i-1 (bit 0) and i+1 (bit 2): vertical neighbours U and D
+    // (significant, negative) of the vertical pair U, D (own column rows i-1, i+1)
+    // and of the horizontal pair: neighbour lane's column and own other column at row i
     const u32 gc = FIRST ? ((u32)(s.G[c] << 1) & 5u) : ((u32)(s.G[c] >> (i - 1)) & 5u);
-    const int chi_u = chi(wc & 1u, gc & 1u);
-    const int chi_d = chi((wc >> 2) & 1u, (gc >> 2) & 1u);
-    const int chi_n = chi((nb >> 1) & 1u, (nb >> 3) & 1u);
-    const int chi_o = chi((u32)(s.S[o] >> i) & 1u, (u32)(s.G[o] >> i) & 1u);
-    const int k = 3 * sgn3(chi_n + chi_o) + sgn3(chi_u + chi_d);
-    const u32 sctx = (u32)(SIGN_CTX_TABLE >> (4 * k)) & 0xFu;
+    const u32 vidx = wc | ((gc & wc) << 1);
+    const u32 so = (wo >> 1) & 1u, go = (u32)(s.G[o] >> i) & so;
+    const u32 sn = (nb >> 1) & 1u, gn = (nb >> 3) & sn;
+    const u32 hidx = sn | (gn << 1) | (so << 2) | (go << 3);
+    const u32 vs = (PAIR_SIGN_TABLE >> (2 * vidx)) & 3u;
+    const u32 hs = (PAIR_SIGN_TABLE >> (2 * hidx)) & 3u;
+    const u32 sctx = (u32)(SIGN_CTX_TABLE >> (4 * (3 * hs + vs))) & 0xFu;
     reserve<DEC>(s, nsig, g, src);
     if (nsig) {
       const u32 prob = ((CLEAN ? pl.cpsign : pl.sign) >> (8u * (sctx >> 1))) & 127u;
@@ -457,8 +470,10 @@ __global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
   if (!DEC) {
     // flush of each lane's open codeword (BPC_PaCo.cu:1020)
     if (good && s.slot < (u32)a.stride) a.dst[g.region + s.slot] = (int32_t)s.L;
-    if (good && a.fill)
-      for (u32 j = cnt + hl; j < (u32)a.stride; j += 32) a.dst[g.region + j] = -1;
+    // -1 in every word past the used slots (BPC_PaCo.cu:1284); a codeblock that was
+    // not coded (all zero, or rejected) keeps only its MSB word
+    if (valid && a.fill)
+      for (u32 j = (good ? cnt : 1u) + hl; j < (u32)a.stride; j += 32) a.dst[g.region + j] = -1;
     if (valid && hl == 0 && a.ncw) a.ncw[cb] = good ? (int32_t)(cnt - 1) : 0;
   } else {
     // Write_Coefficients (BPC_PaCo.cu:46-63): magnitude from the decoded planes

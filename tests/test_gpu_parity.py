@@ -128,13 +128,14 @@ def test_odd_codeblock_count_and_zero_blocks(bpc, O, cp3):
 @pytest.mark.parametrize("cp3", [0, 1])
 def test_wide_magnitudes(bpc, O, cp3):
     """Magnitudes up to 2^27 - 1 (MSB 27, the largest the word format holds) with a
-    28-bitplane LUT: the 28-plane kernel variant."""
+    28-bitplane LUT: the 28-plane kernel variant.  ~27 bits per sample need more than
+    the reference's 4095 codeword slots, so the regions are 16384 words."""
     rng = np.random.default_rng(9)
     img = rng.integers(-(1 << 27) + 1, 1 << 27, (128, 128)).astype(np.int32)
     img[64:, 64:] >>= 20
     p = bpc.params(cpass3=cp3, dwt_levels=1, lut_n_bitplanes=28)
     lut = bpc.train_lut(img, p)
-    check_codec(bpc, O, img, lut, cp3, levels=1, nbp=28)
+    check_codec(bpc, O, img, lut, cp3, levels=1, nbp=28, stride=16384)
 
 
 def test_msb_beyond_lut_planes_aliases_like_reference(bpc, O):
@@ -144,21 +145,22 @@ def test_msb_beyond_lut_planes_aliases_like_reference(bpc, O):
     rng = np.random.default_rng(10)
     img = rng.integers(-(1 << 17), 1 << 17, (64, 128)).astype(np.int32)  # MSB 17 in both codeblocks
     img[:, :64] >>= 12  # codeblock 0 (LL) MSB 5
-    p = bpc.params(cpass3=0, dwt_levels=1, lut_n_bitplanes=15)
+    p = bpc.params(cpass3=0, dwt_levels=1, lut_n_bitplanes=15, cb_stride_words=16384)
     lut_arr = rng.integers(1, 128, bpc.lut_len(p)).astype(np.int32)
-    op = O.params(cpass3=0, dwt_levels=1, lut_n_bitplanes=15)
+    op = O.params(cpass3=0, dwt_levels=1, lut_n_bitplanes=15, cb_stride_words=16384)
     # codeblock 1 is level-0 HL (group 0): rows up to 17 stay inside the LUT array
     st, ostream, _ = O.encode(img, lut_arr, op)
     assert st == 0
     lut = bpc.Lut.from_array(lut_arr)
     try:
-        buf = img.copy().ravel()
+        buf = np.zeros(2 * 16384, np.int32)
+        buf[:img.size] = img.ravel()
         bpc.run(bpc.CODE, buf, 128, 64, lut, p)
         assert np.array_equal(buf, ostream)
         bpc.run(bpc.DECODE, buf, 128, 64, lut, p)
-        assert np.array_equal(buf.reshape(64, 128), img)
+        assert np.array_equal(buf[:img.size].reshape(64, 128), img)
         d_in = bpc.DeviceBuffer.from_array(img.ravel())
-        d_cs = bpc.DeviceBuffer(128 * 64)
+        d_cs = bpc.DeviceBuffer(2 * 16384)
         bpc.encode_device(d_in, d_cs, 128, 64, lut, p)
         assert bpc.status_take() == bpc.ERR_WIDE
     finally:
