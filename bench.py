@@ -71,6 +71,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle baseline and parity check")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB 5 times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every codeblock is independent and a multi-GPU run "
@@ -79,6 +81,16 @@ def main():
     if bpc.device_count() < 1:
         print("no HIP device", file=sys.stderr)
         sys.exit(1)
+
+    if args.calibrate_copy:
+        n = args.calibrate_copy << 20
+        a, b = bpc.DeviceBuffer(n // 4), bpc.DeviceBuffer(n // 4)
+        a.fill_bytes(1)
+        for _ in range(5):
+            b.copy_from(a)
+        bpc.device_sync()
+        print(json.dumps({"calibration_copy_bytes": n, "launches": 5}))
+        return
 
     size, cp3, nbp = 4096, 1, 16
     log("workload plain_4096_3pass: generating image and training LUT")

@@ -101,6 +101,7 @@ def lib():
             "bpc_dev_memset": (i32, [P, i32, sz, P]),
             "bpc_copy_h2d": (i32, [P, P, sz, P]),
             "bpc_copy_d2h": (i32, [P, P, sz, P]),
+            "bpc_copy_d2d": (i32, [P, P, sz, P]),
             "bpc_stream_create": (i32, [ctypes.POINTER(P)]),
             "bpc_stream_destroy": (i32, [P]),
             "bpc_stream_sync": (i32, [P]),
@@ -171,6 +172,11 @@ class DeviceBuffer:
         out = np.empty(n, dtype=self.dtype)
         _check(lib().bpc_copy_d2h(_ptr(out), self.ptr, out.nbytes, _sp(stream)))
         return out
+
+    def copy_from(self, other, nbytes=None, stream=None):
+        """Asynchronous device-to-device copy from another DeviceBuffer."""
+        nb = other.nbytes if nbytes is None else nbytes
+        _check(lib().bpc_copy_d2d(self.ptr, other.ptr, nb, _sp(stream)))
 
     def fill_bytes(self, value, stream=None):
         _check(lib().bpc_dev_memset(self.ptr, value, self.nbytes, _sp(stream)))

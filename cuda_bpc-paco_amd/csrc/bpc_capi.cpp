@@ -578,6 +578,11 @@ int bpc_copy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
   HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
   return BPC_OK;
 }
+int bpc_copy_d2d(void* dst, const void* src, size_t bytes, void* stream) {
+  hipError_t e = bpc::launch_copy(dst, src, bytes, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "device copy (16-byte aligned, multiple of 16 bytes)");
+  return BPC_OK;
+}
 int bpc_stream_create(void** stream) {
   hipStream_t s;
   HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));

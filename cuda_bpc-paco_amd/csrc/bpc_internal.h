@@ -35,5 +35,7 @@ struct KArgs {
 // Kernel variants hold this many bitplanes in registers.
 int planes_for(int nbp);
 hipError_t launch_codec(int decode, int cpass3, int maxp, const KArgs& a, hipStream_t stream);
+// 16-byte aligned device-to-device copy of a multiple of 16 bytes.
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
 
 }  // namespace bpc

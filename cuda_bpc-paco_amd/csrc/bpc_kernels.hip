@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "bpc_internal.h"
 
 namespace bpc {
@@ -498,6 +500,21 @@ __global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
 }
 
 }  // namespace
+
+// Device-to-device copy, 16 bytes per lane per iteration (grid-stride).
+__global__ __launch_bounds__(256) void copy_kernel(const int4* __restrict__ src, int4* __restrict__ dst, size_t n16) {
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n16; k += (size_t)gridDim.x * blockDim.x)
+    dst[k] = src[k];
+}
+
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t stream) {
+  if (bytes % 16 || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15)) return hipErrorInvalidValue;
+  const size_t n16 = bytes / 16;
+  if (!n16) return hipSuccess;
+  const size_t blocks = std::min<size_t>((n16 + 255) / 256, 256 * 64);
+  hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, (const int4*)src, (int4*)dst, n16);
+  return hipGetLastError();
+}
 
 int planes_for(int nbp) {
   if (nbp <= 16) return 16;

@@ -116,6 +116,9 @@ int bpc_dev_free(void* ptr);
 int bpc_dev_memset(void* ptr, int value, size_t bytes, void* stream);
 int bpc_copy_h2d(void* dst, const void* src, size_t bytes, void* stream);
 int bpc_copy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+/* Asynchronous device-to-device copy by a plain copy kernel; 16-byte aligned
+ * pointers, bytes a multiple of 16. */
+int bpc_copy_d2d(void* dst, const void* src, size_t bytes, void* stream);
 int bpc_stream_create(void** stream);
 int bpc_stream_destroy(void* stream);
 int bpc_stream_sync(void* stream);
