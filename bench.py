@@ -39,6 +39,23 @@ def make_workload(size, cp3, nbp, mag_bits, ll_bits, detail_exp, seed, levels=5)
     return img, lut, p
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary
+    of this workload (FETCH_SIZE/WRITE_SIZE scaled by a plain copy), when it was
+    taken on the current kernel source; otherwise None."""
+    import hashlib
+    path = os.path.join(ROOT, "profiles", "bpc_codec_%s.traffic.json" % workload)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    with open(os.path.join(ROOT, "cuda_bpc-paco_amd", "csrc", "bpc_kernels.hip"), "rb") as f:
+        sha = hashlib.sha256(f.read()).hexdigest()
+    if rec.get("kernel_source_sha256") != sha:
+        return None
+    return rec["traffic_bytes_per_launch"].get(kernel)
+
+
 def cpu_baseline(img, lut_arr, p, cp3, nbp, budget_s=20.0):
     """Oracle (C, OpenMP over codeblocks) encode+decode of the same image on the host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -183,7 +200,8 @@ def main():
         "bits_per_sample": round(float(ncw.sum()) * 16 / samples, 4),
         "roofline": {"bound": "hbm", "kernel": "bpc_codec_kernel (%s)" % dom_name,
                      "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 6),
+                     "traffic": measured_traffic("plain_4096_3pass", dom_name),
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": None,
         "bit_exact_vs_oracle": None,
