@@ -60,6 +60,21 @@ class Params(ctypes.Structure):
         return (size_x // 64) * (size_y // self.cblock_length)
 
 
+class Image(ctypes.Structure):
+    """bpc_image: one image of a batch (device pointers)."""
+    _fields_ = [
+        ("src", ctypes.c_void_p),
+        ("dst", ctypes.c_void_p),
+        ("lut", ctypes.c_void_p),
+        ("lut_len", ctypes.c_int32),
+        ("ncw", ctypes.c_void_p),
+        ("size_x", ctypes.c_int32),
+        ("size_y", ctypes.c_int32),
+        ("params", Params),
+        ("fill_unused", ctypes.c_int32),
+    ]
+
+
 def params(cpass3=0, dwt_levels=5, cblock_length=64, lut_n_bitplanes=15, cb_stride_words=0, layout=LAYOUT_RASTER):
     return Params(cpass3, dwt_levels, cblock_length, lut_n_bitplanes, cb_stride_words, layout)
 
@@ -111,6 +126,10 @@ def lib():
             "bpc_event_record": (i32, [P, P]),
             "bpc_event_elapsed_ms": (i32, [P, P, ctypes.POINTER(ctypes.c_float)]),
             "bpc_mem_info": (i32, [ctypes.POINTER(sz), ctypes.POINTER(sz)]),
+            "bpc_batch_create": (i32, [ctypes.POINTER(P), i32, ctypes.c_longlong]),
+            "bpc_batch_destroy": (i32, [P]),
+            "bpc_batch_set_tuning": (i32, [P, i32, i32]),
+            "bpc_batch_run": (i32, [P, i32, ctypes.POINTER(Image), i32, i32, P, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -373,3 +392,42 @@ def decode(stream_words_arr, size_x, size_y, lut, p):
 
 def _tiled_shape(img, p):
     raise ValueError("tiled input needs explicit sizes; use encode_device")
+
+
+# ---- batched codec -------------------------------------------------------------------
+
+class Batch:
+    """A device work queue coding many images of uneven size in one persistent launch."""
+
+    def __init__(self, max_images, max_codeblocks):
+        p = ctypes.c_void_p()
+        _check(lib().bpc_batch_create(ctypes.byref(p), max_images, max_codeblocks))
+        self.ptr = p
+
+    def set_tuning(self, waves_per_block=4, waves_per_simd=0):
+        _check(lib().bpc_batch_set_tuning(self.ptr, waves_per_block, waves_per_simd))
+
+    def run(self, direction, images, use_graph=False, d_status=None, stream=None):
+        """images: list of Image (or of dicts with the Image fields)."""
+        arr = (Image * len(images))()
+        for k, im in enumerate(images):
+            arr[k] = im if isinstance(im, Image) else make_image(**im)
+        _check(lib().bpc_batch_run(self.ptr, direction, arr, len(images), 1 if use_graph else 0,
+                                   d_status.ptr if d_status is not None else None, _sp(stream)))
+
+    def destroy(self):
+        if self.ptr is not None:
+            lib().bpc_batch_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def make_image(src, dst, lut, size_x, size_y, p, ncw=None, fill_unused=1):
+    """Image descriptor from DeviceBuffers and a Lut."""
+    return Image(src.ptr, dst.ptr, lut.ptr, lut.len, ncw.ptr if ncw is not None else None, size_x, size_y, p,
+                 fill_unused)

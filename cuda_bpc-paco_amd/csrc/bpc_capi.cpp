@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <atomic>
 #include <mutex>
 #include <thread>
@@ -622,6 +623,252 @@ int bpc_event_elapsed_ms(void* start, void* stop, float* ms) {
 }
 int bpc_mem_info(size_t* free_bytes, size_t* total_bytes) {
   HIP_TRY(hipMemGetInfo(free_bytes, total_bytes));
+  return BPC_OK;
+}
+
+}  // extern "C"
+
+// ---- batched codec ---------------------------------------------------------
+
+struct bpc_batch {
+  int device = 0;
+  int max_images = 0;
+  long long max_cbs = 0;
+  bpc::BatchImage* d_images = nullptr;
+  bpc::BatchImage* h_images = nullptr;  // pinned staging for the descriptor upload
+  long long* d_offsets = nullptr;       // cb_offset[n+1], sorted_base[n]
+  long long* h_offsets = nullptr;
+  uint32_t* d_keys = nullptr;
+  uint32_t* d_sorted = nullptr;
+  uint32_t* d_pair_keys = nullptr;
+  uint32_t* d_pair_queue = nullptr;
+  uint32_t* d_ctl = nullptr;
+  hipEvent_t staged = nullptr;  // the last descriptor upload has been consumed
+  bool staged_pending = false;
+  int waves_per_block = 4;
+  int waves_per_simd = 0;  // 0: as many as fit
+  int cus = 0;
+  std::map<std::string, hipGraphExec_t> graphs;
+  std::mutex mu;
+};
+
+namespace {
+
+void batch_release(bpc_batch* b) {
+  for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+  b->graphs.clear();
+  void* dev[] = {b->d_images, b->d_offsets, b->d_keys, b->d_sorted, b->d_pair_keys, b->d_pair_queue, b->d_ctl};
+  for (void* p : dev)
+    if (p) (void)hipFree(p);
+  if (b->h_images) (void)hipHostFree(b->h_images);
+  if (b->h_offsets) (void)hipHostFree(b->h_offsets);
+  if (b->staged) (void)hipEventDestroy(b->staged);
+}
+
+struct BatchPlan {
+  long long total_cbs = 0, total_pairs = 0;
+  int maxp = 16;
+  int rows = 0;
+  std::string signature;
+};
+
+int batch_stage(bpc_batch* b, int direction, const bpc_image* images, int n, BatchPlan& plan) {
+  if (n <= 0 || n > b->max_images) return fail(BPC_ERR_ARG, "batch of %d images (capacity %d)", n, b->max_images);
+  if (n > 4096) return fail(BPC_ERR_ARG, "at most 4096 images per batch");
+  plan.rows = images[0].params.cblock_length;
+  int max_nbp = 1;
+  char sig[128];
+  plan.signature = std::to_string(direction) + ":";
+  long long pos = 0;
+  for (int k = 0; k < n; ++k) {
+    const bpc_image& im = images[k];
+    int st = check_params(&im.params);
+    if (st) return st;
+    if (im.params.cblock_length != plan.rows) return fail(BPC_ERR_ARG, "all images of a batch share cblock_length");
+    st = check_image(im.size_x, im.size_y, &im.params);
+    if (st) return st;
+    if (!im.src || !im.dst || !im.lut) return fail(BPC_ERR_ARG, "image %d: null buffer", k);
+    if (((uintptr_t)im.src & 7) || ((uintptr_t)im.dst & 7)) return fail(BPC_ERR_ARG, "image %d: buffers must be 8-byte aligned", k);
+    if (im.lut_len < bpc_lut_len(&im.params)) return fail(BPC_ERR_ARG, "image %d: lut_len too small", k);
+    const long long ncb = (long long)(im.size_x / 64) * (im.size_y / plan.rows);
+    if (ncb > (1 << 20)) return fail(BPC_ERR_ARG, "image %d: more than 2^20 codeblocks", k);
+    bpc::BatchImage& d = b->h_images[k];
+    d.src = im.src;
+    d.dst = im.dst;
+    d.lut = im.lut;
+    d.ncw = direction == BPC_CODE ? im.ncw : nullptr;
+    d.size_x = im.size_x;
+    d.size_y = im.size_y;
+    d.nx = im.size_x / 64;
+    d.ncb = (int32_t)ncb;
+    d.stride = stride_of(&im.params);
+    d.nbp = im.params.lut_n_bitplanes;
+    d.levels = im.params.dwt_levels;
+    d.lut_len = im.lut_len;
+    d.layout = im.params.layout;
+    d.fill = im.fill_unused;
+    d.cpass3 = im.params.cpass3;
+    d.pad = 0;
+    b->h_offsets[k] = plan.total_cbs;
+    b->h_offsets[n + 1 + k] = pos;
+    plan.total_cbs += ncb;
+    pos += (ncb + 1) & ~1ll;
+    max_nbp = std::max(max_nbp, im.params.lut_n_bitplanes);
+    snprintf(sig, sizeof sig, "%dx%d/%d%d%d%d/%d;", im.size_x, im.size_y, im.params.cpass3, im.params.dwt_levels,
+             im.params.lut_n_bitplanes, im.params.layout, d.stride);
+    plan.signature += sig;
+  }
+  b->h_offsets[n] = plan.total_cbs;
+  plan.total_pairs = pos / 2;
+  if (plan.total_cbs > b->max_cbs) return fail(BPC_ERR_ARG, "batch of %lld codeblocks (capacity %lld)", plan.total_cbs, b->max_cbs);
+  plan.maxp = bpc::planes_for(max_nbp);
+  return BPC_OK;
+}
+
+int batch_enqueue(bpc_batch* b, int direction, int n, const BatchPlan& plan, int maxp, uint32_t* d_status,
+                  hipStream_t stream) {
+  HIP_TRY(hipMemcpyAsync(b->d_images, b->h_images, sizeof(bpc::BatchImage) * n, hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipMemcpyAsync(b->d_offsets, b->h_offsets, sizeof(long long) * (2 * n + 1), hipMemcpyHostToDevice, stream));
+  bpc::QueueArgs q;
+  q.images = b->d_images;
+  q.cb_offset = b->d_offsets;
+  q.sorted_base = b->d_offsets + n + 1;
+  q.n_images = n;
+  q.total_cbs = plan.total_cbs;
+  q.total_pairs = plan.total_pairs;
+  q.rows = plan.rows;
+  q.decode = direction == BPC_DECODE;
+  q.keys = b->d_keys;
+  q.sorted = b->d_sorted;
+  q.pair_keys = b->d_pair_keys;
+  q.pair_queue = b->d_pair_queue;
+  q.ctl = b->d_ctl;
+  hipError_t e = bpc::launch_queue_build(q, stream);
+  if (e != hipSuccess) return hip_fail(e, "queue build");
+  bpc::BatchArgs a;
+  a.images = b->d_images;
+  a.sorted = b->d_sorted;
+  a.pair_queue = b->d_pair_queue;
+  a.ctl = b->d_ctl;
+  a.n_pairs = plan.total_pairs;
+  a.rows = plan.rows;
+  if (plan.total_pairs > 0) {
+    const int wpb = b->waves_per_block;
+    int per_cu = bpc::batch_resident_blocks_per_cu(direction == BPC_DECODE, maxp, wpb);
+    if (per_cu <= 0) per_cu = 1;
+    if (b->waves_per_simd > 0) per_cu = std::max(1, std::min(per_cu, (4 * b->waves_per_simd) / wpb));
+    long long blocks = (long long)per_cu * b->cus;
+    blocks = std::min(blocks, (plan.total_pairs + wpb - 1) / wpb);
+    e = bpc::launch_batch(direction == BPC_DECODE, maxp, a, (int)blocks, wpb, stream);
+    if (e != hipSuccess) return hip_fail(e, "batched kernel launch");
+  }
+  e = bpc::launch_status_fold(b->d_ctl, d_status, stream);
+  if (e != hipSuccess) return hip_fail(e, "status fold");
+  return BPC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bpc_batch_create(bpc_batch** out, int max_images, long long max_codeblocks) {
+  if (!out || max_images <= 0 || max_images > 4096 || max_codeblocks <= 0)
+    return fail(BPC_ERR_ARG, "bad batch capacity");
+  bpc_batch* b = new bpc_batch();
+  b->max_images = max_images;
+  b->max_cbs = max_codeblocks;
+  hipError_t e = hipGetDevice(&b->device);
+  int cus = 0;
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
+  b->cus = cus > 0 ? cus : 1;
+  const size_t entries = (size_t)max_codeblocks + 2 * (size_t)max_images + 2;
+  if (e == hipSuccess) e = hipMalloc(&b->d_images, sizeof(bpc::BatchImage) * max_images);
+  if (e == hipSuccess) e = hipHostMalloc(&b->h_images, sizeof(bpc::BatchImage) * max_images, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc(&b->d_offsets, sizeof(long long) * (2 * max_images + 1));
+  if (e == hipSuccess) e = hipHostMalloc(&b->h_offsets, sizeof(long long) * (2 * max_images + 1), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc(&b->d_keys, sizeof(uint32_t) * max_codeblocks);
+  if (e == hipSuccess) e = hipMalloc(&b->d_sorted, sizeof(uint32_t) * entries);
+  if (e == hipSuccess) e = hipMalloc(&b->d_pair_keys, sizeof(uint32_t) * (entries / 2 + 1));
+  if (e == hipSuccess) e = hipMalloc(&b->d_pair_queue, sizeof(uint32_t) * (entries / 2 + 1));
+  if (e == hipSuccess) e = hipMalloc(&b->d_ctl, sizeof(uint32_t) * (bpc::CTL_IMG + 32 * (size_t)max_images));
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->staged, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    batch_release(b);
+    delete b;
+    return hip_fail(e, "batch buffers");
+  }
+  *out = b;
+  return BPC_OK;
+}
+
+int bpc_batch_destroy(bpc_batch* b) {
+  if (!b) return BPC_OK;
+  if (b->staged_pending) (void)hipEventSynchronize(b->staged);
+  batch_release(b);
+  delete b;
+  return BPC_OK;
+}
+
+int bpc_batch_set_tuning(bpc_batch* b, int waves_per_block, int waves_per_simd) {
+  if (!b || waves_per_block < 1 || waves_per_block > 4 || waves_per_simd < 0 || waves_per_simd > 8)
+    return fail(BPC_ERR_ARG, "bad tuning");
+  std::lock_guard<std::mutex> lk(b->mu);
+  b->waves_per_block = waves_per_block;
+  b->waves_per_simd = waves_per_simd;
+  return BPC_OK;
+}
+
+int bpc_batch_run(bpc_batch* b, int direction, const bpc_image* images, int n_images, int use_graph,
+                  uint32_t* d_status, void* stream) {
+  if (!b || !images) return fail(BPC_ERR_ARG, "null batch or images");
+  if (direction != BPC_CODE && direction != BPC_DECODE) return fail(BPC_ERR_ARG, "bad direction");
+  std::lock_guard<std::mutex> lk(b->mu);
+  if (!d_status) {
+    int st = default_status(&d_status);
+    if (st) return st;
+  }
+  // the pinned staging area may still be read by the previous run's upload
+  if (b->staged_pending) HIP_TRY(hipEventSynchronize(b->staged));
+  BatchPlan plan;
+  int st = batch_stage(b, direction, images, n_images, plan);
+  if (st) return st;
+  hipStream_t s = (hipStream_t)stream;
+  const int maxp = plan.maxp;
+  if (!use_graph) {
+    st = batch_enqueue(b, direction, n_images, plan, maxp, d_status, s);
+  } else {
+    char tun[64];
+    snprintf(tun, sizeof tun, "|%d,%d,%d,%p", b->waves_per_block, b->waves_per_simd, maxp, (void*)d_status);
+    const std::string key = plan.signature + tun;
+    auto it = b->graphs.find(key);
+    if (it == b->graphs.end()) {
+      hipStream_t cap = s;
+      bool own = false;
+      if (!cap) {  // the null stream cannot be captured
+        HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        own = true;
+      }
+      HIP_TRY(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+      st = batch_enqueue(b, direction, n_images, plan, maxp, d_status, cap);
+      hipGraph_t graph = nullptr;
+      hipError_t e = hipStreamEndCapture(cap, &graph);
+      if (own) (void)hipStreamDestroy(cap);
+      if (st) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+      }
+      if (e != hipSuccess) return hip_fail(e, "graph capture");
+      hipGraphExec_t exec = nullptr;
+      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) return hip_fail(e, "graph instantiate");
+      it = b->graphs.emplace(key, exec).first;
+    }
+    HIP_TRY(hipGraphLaunch(it->second, s));
+  }
+  if (st) return st;
+  HIP_TRY(hipEventRecord(b->staged, s));
+  b->staged_pending = true;
   return BPC_OK;
 }
 

@@ -32,9 +32,57 @@ struct KArgs {
   int fill;            // encoder: write -1 to unused region words
 };
 
+// One image of a batch, as the device sees it.
+struct BatchImage {
+  const int32_t* src;
+  int32_t* dst;
+  const int32_t* lut;
+  int32_t* ncw;
+  int32_t size_x, size_y, nx, ncb, stride, nbp, levels, lut_len, layout, fill, cpass3, pad;
+};
+
+// Control words of a batch queue (device memory): CTL_IMG + 32*n_images words.
+enum : int {
+  CTL_PAIRHIST = 0,  // 58 pair-bucket counts, then cursors
+  CTL_HEAD = 64,     // next queue slot to hand out
+  CTL_STATUS = 65,   // status bits of the batch
+  CTL_IMG = 128,     // per image: 29 MSB-bucket counts, then cursors (32 words each)
+};
+
+struct QueueArgs {
+  const BatchImage* images;
+  const long long* cb_offset;    // first global codeblock of each image (n_images + 1 entries)
+  const long long* sorted_base;  // first entry of each image in `sorted` (even-padded ranges)
+  int n_images;
+  long long total_cbs;
+  long long total_pairs;
+  int rows;
+  int decode;
+  uint32_t* keys;        // per global codeblock: MSB bucket
+  uint32_t* sorted;      // 2*total_pairs entries: image << 20 | codeblock, sorted by MSB inside each image
+  uint32_t* pair_keys;   // per pair: pass mode * 29 + bucket
+  uint32_t* pair_queue;  // pair indices, costliest first
+  uint32_t* ctl;
+};
+
+struct BatchArgs {
+  const BatchImage* images;
+  const uint32_t* sorted;
+  const uint32_t* pair_queue;
+  uint32_t* ctl;
+  long long n_pairs;
+  int rows;
+};
+
 // Kernel variants hold this many bitplanes in registers.
 int planes_for(int nbp);
 hipError_t launch_codec(int decode, int cpass3, int maxp, const KArgs& a, hipStream_t stream);
+hipError_t launch_queue_build(const QueueArgs& a, hipStream_t stream);
+hipError_t launch_batch(int decode, int maxp, const BatchArgs& a, int grid_blocks, int waves_per_block,
+                        hipStream_t stream);
+hipError_t launch_status_fold(uint32_t* ctl, uint32_t* status, hipStream_t stream);
+// Workgroups of the batched kernel that fit on one CU (occupancy query).
+int batch_resident_blocks_per_cu(int decode, int maxp, int waves_per_block);
 // 16-byte aligned device-to-device copy of a multiple of 16 bytes.
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
 

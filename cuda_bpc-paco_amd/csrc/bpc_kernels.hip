@@ -283,29 +283,40 @@ __device__ __forceinline__ PlaneLut load_plane_lut(const int32_t* lut, int ref0,
   return pl;
 }
 
+// The image one half-wave codes: wave-uniform in the per-image kernel,
+// half-uniform in the batched kernel (the halves may belong to different images).
+struct HalfDesc {
+  const int32_t* src;
+  int32_t* dst;
+  const int32_t* lut;
+  int32_t* ncw;
+  int size_x, size_y, nx, stride, nbp, levels, lut_len, layout, fill;
+  int cb;  // codeblock of this half in its image, -1 for an empty half
+};
+
+// Codes the two codeblocks of one wave (Kernel_BPC_CODER / Kernel_BPC_DECODER
+// per warp, BPC_PaCo.cu:1097-1224).
 template <int DEC, int CP3, int MAXP>
-__global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
+__device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uint32_t* status_word) {
   const int lane = threadIdx.x & 63;
   const int half = lane >> 5;
   const int hl = lane & 31;
-  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  const int cb = 2 * wave + half;
-  const bool valid = cb < a.ncb;
-  const int cbc = valid ? cb : a.ncb - 1;
+  const int cb = a.cb;
+  const bool valid = cb >= 0;
+  const int cbc = valid ? cb : 0;
   const int tx = cbc % a.nx, ty = cbc / a.nx;
   const int x = tx * 64 + 2 * hl;  // TCoordinate_X, BPC_PaCo.cu:1126
-  const int y0 = ty * a.rows;      // TCoordinate_Y, BPC_PaCo.cu:1127
-
+  const int y0 = ty * rows;        // TCoordinate_Y, BPC_PaCo.cu:1127
   Geo g;
   g.half = half;
   g.hl = hl;
   g.region = (int64_t)cbc * a.stride;  // WCoordinate, BPC_PaCo.cu:1130
   g.stride = a.stride;
-  g.rows = a.rows;
+  g.rows = rows;
   int64_t cbase;
   int rs;
   if (a.layout) {
-    cbase = (int64_t)cbc * 64 * a.rows + 2 * hl;
+    cbase = (int64_t)cbc * 64 * rows + 2 * hl;
     rs = 64;
   } else {
     cbase = (int64_t)y0 * a.size_x + x;
@@ -326,7 +337,7 @@ __global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
   s.cnt0 = 1;
   s.cnt1 = 1;
   if (CP3) {  // every coefficient starts with the cleanup flag (BPC_PaCo.cu:34-39, 73-78)
-    const u64 rm = a.rows >= 64 ? ~0ull : ((1ull << a.rows) - 1ull);
+    const u64 rm = rows >= 64 ? ~0ull : ((1ull << rows) - 1ull);
     s.C[0] = s.C[1] = rm;
   }
 
@@ -336,8 +347,9 @@ __global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
   if (!DEC) {
     // Read_Coefficients (BPC_PaCo.cu:14-44) straight into bit-sliced planes.
     u32 mor = 0;
+#pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const int r0 = 32 * h, r1 = min(a.rows, 32 * h + 32);
+      const int r0 = 32 * h, r1 = min(rows, 32 * h + 32);
       u32 plo[MAXP][2];
 #pragma unroll
       for (int p = 0; p < MAXP; ++p) plo[p][0] = plo[p][1] = 0;
@@ -481,7 +493,7 @@ __global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
     // Write_Coefficients (BPC_PaCo.cu:46-63): magnitude from the decoded planes
     const bool emit = good && !overflow;
     if (valid) {
-      for (int r = 0; r < a.rows; ++r) {
+      for (int r = 0; r < rows; ++r) {
         u32 m0 = 0, m1 = 0;
 #pragma unroll
         for (int p = 0; p < MAXP; ++p) {
@@ -496,9 +508,177 @@ __global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
       }
     }
   }
-  if (status) atomicOr(a.status, status);
+  if (status) atomicOr(status_word, status);
 }
 
+
+template <int DEC, int CP3, int MAXP>
+__global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int cb = 2 * wave + ((threadIdx.x & 63) >> 5);
+  HalfDesc d;
+  d.src = a.src;
+  d.dst = a.dst;
+  d.lut = a.lut;
+  d.ncw = a.ncw;
+  d.size_x = a.size_x;
+  d.size_y = a.size_y;
+  d.nx = a.nx;
+  d.stride = a.stride;
+  d.nbp = a.nbp;
+  d.levels = a.levels;
+  d.lut_len = a.lut_len;
+  d.layout = a.layout;
+  d.fill = a.fill;
+  d.cb = cb < a.ncb ? cb : -1;
+  code_pair<DEC, CP3, MAXP>(d, a.rows, a.status);
+}
+
+// ---- batched path ---------------------------------------------------------
+// Codeblocks are sorted by MSB inside each image and paired there (both halves of
+// a wave always code the same image, so its descriptor is wave-uniform); pairs
+// are then ordered by (pass mode, MSB) with the costliest first.
+constexpr u32 EMPTY = 0xFFFFFFFFu;
+constexpr int NBUCKET = 29;  // MSB 27..0, then all-zero / rejected
+
+__device__ __forceinline__ u32 msb_bucket(int msb) { return (msb >= 0 && msb <= 27) ? (u32)(27 - msb) : (u32)(NBUCKET - 1); }
+
+// Persistent kernel: every wave takes the next pair off the queue until it is drained.
+template <int DEC, int MAXP>
+__global__ __launch_bounds__(256) void bpc_batch_kernel(const BatchArgs a) {
+  const int half = (threadIdx.x & 63) >> 5;
+  for (;;) {
+    u32 slot = 0;
+    if ((threadIdx.x & 63) == 0) slot = atomicAdd(&a.ctl[CTL_HEAD], 1u);
+    slot = __builtin_amdgcn_readfirstlane(slot);
+    if (slot >= (u32)a.n_pairs) break;
+    const u32 pair = __builtin_amdgcn_readfirstlane(a.pair_queue[slot]);
+    const u32 first = __builtin_amdgcn_readfirstlane(a.sorted[2 * (size_t)pair]);
+    const u32 e = a.sorted[2 * (size_t)pair + half];
+    const BatchImage& im = a.images[first >> 20];
+    HalfDesc d;
+    d.src = im.src;
+    d.dst = im.dst;
+    d.lut = im.lut;
+    d.ncw = im.ncw;
+    d.size_x = im.size_x;
+    d.size_y = im.size_y;
+    d.nx = im.nx;
+    d.stride = im.stride;
+    d.nbp = im.nbp;
+    d.levels = im.levels;
+    d.lut_len = im.lut_len;
+    d.layout = im.layout;
+    d.fill = im.fill;
+    d.cb = e == EMPTY ? -1 : (int)(e & 0xFFFFFu);
+    if (im.cpass3)
+      code_pair<DEC, 1, MAXP>(d, a.rows, &a.ctl[CTL_STATUS]);
+    else
+      code_pair<DEC, 0, MAXP>(d, a.rows, &a.ctl[CTL_STATUS]);
+  }
+}
+
+// Clears the queue counters (kernels rather than memsets, so captured graphs
+// hold no memset nodes).
+__global__ void queue_reset_kernel(u32* ctl, int words) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < words; k += gridDim.x * blockDim.x) ctl[k] = 0;
+}
+
+// One wave per codeblock: MSB bucket, counted per image.
+__global__ __launch_bounds__(256) void queue_keys_kernel(const QueueArgs a) {
+  const long long gcb = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (gcb >= a.total_cbs) return;  // wave-uniform
+  int lo = 0, hi = a.n_images - 1;  // image owning gcb: last k with cb_offset[k] <= gcb
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a.cb_offset[mid] <= gcb) lo = mid; else hi = mid - 1;
+  }
+  const BatchImage& im = a.images[lo];
+  const int cb = (int)(gcb - a.cb_offset[lo]);
+  int msb;
+  if (!a.decode) {
+    const int tx = cb % im.nx, ty = cb / im.nx;
+    const int64_t base = im.layout ? (int64_t)cb * 64 * a.rows : (int64_t)ty * a.rows * im.size_x + tx * 64;
+    const int rs = im.layout ? 64 : im.size_x;
+    u32 m = 0;
+    for (int r = 0; r < a.rows; ++r) {
+      const int v = im.src[base + (int64_t)r * rs + lane];
+      m |= v < 0 ? 0u - (u32)v : (u32)v;
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) m |= (u32)__shfl_xor((int)m, off, 64);
+    msb = m ? 31 - __clz(m) : 32;
+  } else {
+    msb = im.src[(int64_t)cb * im.stride];
+  }
+  if (lane == 0) {
+    const u32 b = msb_bucket(msb);
+    a.keys[gcb] = b;
+    atomicAdd(&a.ctl[CTL_IMG + lo * 32 + b], 1u);
+  }
+}
+
+// One thread per image: bucket cursors inside the image's (even-padded) range.
+__global__ void queue_image_scan_kernel(const QueueArgs a) {
+  const int img = blockIdx.x * blockDim.x + threadIdx.x;
+  if (img >= a.n_images) return;
+  u32 pos = (u32)a.sorted_base[img];
+  for (int b = 0; b < NBUCKET; ++b) {
+    u32* c = &a.ctl[CTL_IMG + img * 32 + b];
+    const u32 n = *c;
+    *c = pos;
+    pos += n;
+  }
+  const long long ncb = a.cb_offset[img + 1] - a.cb_offset[img];
+  if (ncb & 1) a.sorted[pos] = EMPTY;
+}
+
+__global__ __launch_bounds__(256) void queue_scatter_kernel(const QueueArgs a) {
+  const long long gcb = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gcb >= a.total_cbs) return;
+  int lo = 0, hi = a.n_images - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a.cb_offset[mid] <= gcb) lo = mid; else hi = mid - 1;
+  }
+  const u32 pos = atomicAdd(&a.ctl[CTL_IMG + lo * 32 + a.keys[gcb]], 1u);
+  a.sorted[pos] = ((u32)lo << 20) | (u32)(gcb - a.cb_offset[lo]);
+}
+
+// One thread per pair: key (pass mode, bucket of its costlier codeblock).
+__global__ __launch_bounds__(256) void queue_pair_keys_kernel(const QueueArgs a) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.total_pairs) return;
+  const u32 e = a.sorted[2 * p];
+  const int img = (int)(e >> 20);
+  const u32 b = a.keys[a.cb_offset[img] + (e & 0xFFFFFu)];
+  const u32 k = (a.images[img].cpass3 ? NBUCKET : 0) + b;
+  a.pair_keys[p] = k;
+  atomicAdd(&a.ctl[CTL_PAIRHIST + k], 1u);
+}
+
+__global__ void queue_pair_scan_kernel(const QueueArgs a) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  u32 pos = 0;
+  for (int k = 0; k < 2 * NBUCKET; ++k) {
+    const u32 n = a.ctl[CTL_PAIRHIST + k];
+    a.ctl[CTL_PAIRHIST + k] = pos;
+    pos += n;
+  }
+}
+
+__global__ __launch_bounds__(256) void queue_pair_scatter_kernel(const QueueArgs a) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.total_pairs) return;
+  const u32 pos = atomicAdd(&a.ctl[CTL_PAIRHIST + a.pair_keys[p]], 1u);
+  a.pair_queue[pos] = (u32)p;
+}
+
+// Folds the batch's status into the caller's status word.
+__global__ void status_fold_kernel(u32* ctl, uint32_t* status) {
+  if (threadIdx.x == 0 && ctl[CTL_STATUS]) atomicOr(status, ctl[CTL_STATUS]);
+}
 }  // namespace
 
 // Device-to-device copy, 16 bytes per lane per iteration (grid-stride).
@@ -537,6 +717,67 @@ static hipError_t launch_p(int maxp, const KArgs& a, hipStream_t stream) {
     case 28: return launch_one<DEC, CP3, 28>(a, stream);
     default: return hipErrorInvalidValue;
   }
+}
+
+template <int DEC, int MAXP>
+static hipError_t launch_batch_one(const BatchArgs& a, int grid_blocks, int waves_per_block, hipStream_t stream) {
+  hipLaunchKernelGGL((bpc_batch_kernel<DEC, MAXP>), dim3(grid_blocks), dim3(64 * waves_per_block), 0, stream, a);
+  return hipGetLastError();
+}
+
+template <int DEC, int MAXP>
+static int resident_one(int waves_per_block) {
+  int blocks = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, bpc_batch_kernel<DEC, MAXP>, 64 * waves_per_block, 0) !=
+      hipSuccess)
+    return 0;
+  return blocks;
+}
+
+int batch_resident_blocks_per_cu(int decode, int maxp, int waves_per_block) {
+  switch (maxp * 2 + decode) {
+    case 32: return resident_one<0, 16>(waves_per_block);
+    case 33: return resident_one<1, 16>(waves_per_block);
+    case 40: return resident_one<0, 20>(waves_per_block);
+    case 41: return resident_one<1, 20>(waves_per_block);
+    case 56: return resident_one<0, 28>(waves_per_block);
+    case 57: return resident_one<1, 28>(waves_per_block);
+    default: return 0;
+  }
+}
+
+hipError_t launch_batch(int decode, int maxp, const BatchArgs& a, int grid_blocks, int waves_per_block,
+                        hipStream_t stream) {
+  if (waves_per_block < 1 || waves_per_block > 4 || grid_blocks < 1) return hipErrorInvalidValue;
+  switch (maxp * 2 + decode) {
+    case 32: return launch_batch_one<0, 16>(a, grid_blocks, waves_per_block, stream);
+    case 33: return launch_batch_one<1, 16>(a, grid_blocks, waves_per_block, stream);
+    case 40: return launch_batch_one<0, 20>(a, grid_blocks, waves_per_block, stream);
+    case 41: return launch_batch_one<1, 20>(a, grid_blocks, waves_per_block, stream);
+    case 56: return launch_batch_one<0, 28>(a, grid_blocks, waves_per_block, stream);
+    case 57: return launch_batch_one<1, 28>(a, grid_blocks, waves_per_block, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_queue_build(const QueueArgs& a, hipStream_t stream) {
+  const int words = CTL_IMG + 32 * a.n_images;
+  hipLaunchKernelGGL(queue_reset_kernel, dim3((words + 255) / 256), dim3(256), 0, stream, a.ctl, words);
+  if (a.total_cbs > 0) {
+    hipLaunchKernelGGL(queue_keys_kernel, dim3((unsigned)((a.total_cbs + 3) / 4)), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(queue_image_scan_kernel, dim3((a.n_images + 63) / 64), dim3(64), 0, stream, a);
+    hipLaunchKernelGGL(queue_scatter_kernel, dim3((unsigned)((a.total_cbs + 255) / 256)), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(queue_pair_keys_kernel, dim3((unsigned)((a.total_pairs + 255) / 256)), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(queue_pair_scan_kernel, dim3(1), dim3(64), 0, stream, a);
+    hipLaunchKernelGGL(queue_pair_scatter_kernel, dim3((unsigned)((a.total_pairs + 255) / 256)), dim3(256), 0, stream,
+                       a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_status_fold(uint32_t* ctl, uint32_t* status, hipStream_t stream) {
+  hipLaunchKernelGGL(status_fold_kernel, dim3(1), dim3(64), 0, stream, ctl, status);
+  return hipGetLastError();
 }
 
 hipError_t launch_codec(int decode, int cpass3, int maxp, const KArgs& a, hipStream_t stream) {

@@ -110,6 +110,38 @@ int bpc_decode_device(const int32_t* d_stream, int32_t* d_coeffs, int size_x, in
 /* Synchronizes stream, maps the status bits to a BPC_* code and clears them. */
 int bpc_status_take(uint32_t* d_status, void* stream);
 
+/* ---- batched codec (many images of uneven size in one persistent launch) --------
+ * The codeblocks of all images go through one device work queue: sorted by MSB
+ * inside each image and paired there (one wave64 codes two codeblocks of one
+ * image), then ordered by pass mode and cost, costliest first, and drained by a
+ * persistent kernel.  Each image's output is identical to bpc_encode_device /
+ * bpc_decode_device on that image alone.  No counterpart in the reference, which
+ * codes one image per call (CUDA_BPC, BPC_PaCo.cu:1305). */
+typedef struct bpc_image {
+  const int32_t* src; /* encoder: coefficients (params.layout) ; decoder: codestream regions */
+  int32_t* dst;       /* encoder: codestream regions ; decoder: coefficients */
+  const int32_t* lut; /* device LUT of this image */
+  int32_t lut_len;
+  int32_t* ncw;       /* encoder: codeword slots per codeblock (nullable) */
+  int32_t size_x, size_y;
+  bpc_params params;  /* per image; cblock_length must be the same for every image of a batch */
+  int32_t fill_unused; /* encoder: -1 in unused region words */
+} bpc_image;
+
+typedef struct bpc_batch bpc_batch;
+/* Device work-queue buffers for up to max_images images / max_codeblocks codeblocks. */
+int bpc_batch_create(bpc_batch** batch, int max_images, long long max_codeblocks);
+int bpc_batch_destroy(bpc_batch* batch);
+/* Launch shape of the persistent kernel: waves per workgroup (1, 2 or 4, i.e. 2, 4
+ * or 8 codeblocks per workgroup) and resident waves per SIMD (0 = as many as fit). */
+int bpc_batch_set_tuning(bpc_batch* batch, int waves_per_block, int waves_per_simd);
+/* Encodes (BPC_CODE) or decodes (BPC_DECODE) all images, asynchronously on stream.
+ * use_graph != 0 captures the launch sequence into a hipGraph the first time a batch
+ * signature (sizes and parameters of every image, tuning) is seen and replays it
+ * afterwards.  Errors land in d_status (bpc_status_take). */
+int bpc_batch_run(bpc_batch* batch, int direction, const bpc_image* images, int n_images, int use_graph,
+                  uint32_t* d_status, void* stream);
+
 /* ---- device memory / stream / event helpers (thin HIP wrappers) ---- */
 int bpc_dev_malloc(void** ptr, size_t bytes);
 int bpc_dev_free(void* ptr);
