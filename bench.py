@@ -81,6 +81,130 @@ def cpu_baseline(img, lut_arr, p, cp3, nbp, budget_s=20.0):
         enc_stream
 
 
+def percentiles(xs):
+    return [round(float(v), 4) for v in np.percentile(xs, [50, 99])]
+
+
+def bench_batched(args, check=True):
+    """96 images (32 each of 2048^2, 3072^2, 4096^2), 5-level DWT, 14 magnitude
+    bitplanes, a LUT per image trained on its own seeded training image, 2- and
+    3-pass alternating: one persistent launch (graph replay) per direction, against
+    96 plain per-image launches per direction."""
+    from concurrent.futures import ThreadPoolExecutor
+    sizes = [2048] * 32 + [3072] * 32 + [4096] * 32
+    nbp = 15
+
+    def setup(k):
+        size, cp3 = sizes[k], k % 2
+        seed = args.seed + 1000 + 2 * k
+        img = synth.synth_image(size, size, 5, seed, mag_bits=14)
+        train = synth.synth_image(size, size, 5, seed + 1, mag_bits=14)
+        lut = bpc.train_lut(train, bpc.params(cpass3=cp3, lut_n_bitplanes=nbp), nthreads=2)
+        return img, lut
+
+    log("batched_96: generating 96 images and LUTs")
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        data = list(ex.map(setup, range(len(sizes))))
+    items = []
+    for k, (img, lut_arr) in enumerate(data):
+        size = sizes[k]
+        p = bpc.params(cpass3=k % 2, lut_n_bitplanes=nbp, layout=bpc.LAYOUT_TILED)
+        items.append(dict(size=size, p=p, lut=bpc.Lut.from_array(lut_arr),
+                          d_in=bpc.DeviceBuffer.from_array(synth.to_tiled(img)),
+                          d_cs=bpc.DeviceBuffer(bpc.stream_words(size, size, p)),
+                          d_out=bpc.DeviceBuffer(size * size), d_ncw=bpc.DeviceBuffer(p.n_codeblocks(size, size))))
+    total_cbs = sum(it["p"].n_codeblocks(it["size"], it["size"]) for it in items)
+    samples = sum(it["size"] ** 2 for it in items)
+    stream = bpc.Stream()
+    d_status = bpc.DeviceBuffer(1, np.uint32)
+    d_status.fill_bytes(0)
+    bpc.device_sync()
+    batch = bpc.Batch(len(items), total_cbs)
+    enc = [bpc.make_image(it["d_in"], it["d_cs"], it["lut"], it["size"], it["size"], it["p"], it["d_ncw"])
+           for it in items]
+    dec = [bpc.make_image(it["d_cs"], it["d_out"], it["lut"], it["size"], it["size"], it["p"]) for it in items]
+    ev = [bpc.Event() for _ in range(3)]
+
+    def batched_step(graph=True):
+        ev[0].record(stream)
+        batch.run(bpc.CODE, enc, use_graph=graph, d_status=d_status, stream=stream)
+        ev[1].record(stream)
+        batch.run(bpc.DECODE, dec, use_graph=graph, d_status=d_status, stream=stream)
+        ev[2].record(stream)
+        return ev[0].elapsed_ms(ev[1]), ev[1].elapsed_ms(ev[2])
+
+    # autotuner: codeblocks per workgroup (2 per wave) x resident waves per SIMD
+    tune = []
+    for wpb in (1, 2, 4):
+        for wps in (0, 2, 3):
+            batch.set_tuning(wpb, wps)
+            batched_step(graph=False)
+            e, d = batched_step(graph=False)
+            tune.append({"codeblocks_per_workgroup": 2 * wpb, "waves_per_simd": wps or "max",
+                         "encode_ms": round(e, 3), "decode_ms": round(d, 3)})
+    best = min(tune, key=lambda t: t["encode_ms"] + t["decode_ms"])
+    batch.set_tuning(best["codeblocks_per_workgroup"] // 2, 0 if best["waves_per_simd"] == "max" else best["waves_per_simd"])
+    rc = bpc.status_take(d_status, stream)
+    if rc != 0:
+        raise RuntimeError("batched codec status %d: %s" % (rc, bpc.lib().bpc_last_error().decode()))
+    if check:
+        log("batched_96: checking round trips")
+        for k, it in enumerate(items):
+            rec = it["d_out"].download(stream=stream)
+            if not np.array_equal(synth.from_tiled(rec, it["size"], it["size"]), data[k][0]):
+                raise RuntimeError("batched round trip failed on image %d" % k)
+    del data
+    steps = max(2, args.steps // 4)
+    for _ in range(args.warmup):
+        batched_step()
+    lat = []
+    bpc.device_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        e, d = batched_step()
+        lat.append(e + d)
+    bpc.device_sync()
+    wall_b = time.perf_counter() - t0
+    # plain per-image launches, same images
+    def plain_step():
+        ev[0].record(stream)
+        for it in items:
+            bpc.encode_device(it["d_in"], it["d_cs"], it["size"], it["size"], it["lut"], it["p"], 1, it["d_ncw"],
+                              d_status, stream)
+        ev[1].record(stream)
+        for it in items:
+            bpc.decode_device(it["d_cs"], it["d_out"], it["size"], it["size"], it["lut"], it["p"], d_status, stream)
+        ev[2].record(stream)
+        return ev[0].elapsed_ms(ev[1]), ev[1].elapsed_ms(ev[2])
+    for _ in range(args.warmup):
+        plain_step()
+    plat = []
+    bpc.device_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        e, d = plain_step()
+        plat.append(e + d)
+    bpc.device_sync()
+    wall_p = time.perf_counter() - t0
+    rc = bpc.status_take(d_status, stream)
+    if rc != 0:
+        raise RuntimeError("codec status %d" % rc)
+    vb = samples * steps / wall_b / 1e6
+    vp = samples * steps / wall_p / 1e6
+    out = {"workload": "batched_96 (32x2048^2 + 32x3072^2 + 32x4096^2, 2/3-pass alternating, LUT per image)",
+           "images": len(items), "codeblocks": total_cbs, "samples": samples, "steps": steps,
+           "batched_msamples_s": round(vb, 2), "plain_96_launches_msamples_s": round(vp, 2),
+           "batched_vs_plain": round(vb / vp, 3),
+           "batched_step_ms_p50_p99": percentiles(lat), "plain_step_ms_p50_p99": percentiles(plat),
+           "autotune": tune, "chosen": best, "graph_replay": True}
+    batch.destroy()
+    for it in items:
+        for k in ("d_in", "d_cs", "d_out", "d_ncw"):
+            it[k].free()
+        it["lut"].free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +212,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle baseline and parity check")
+    ap.add_argument("--workloads", default="plain,batched",
+                    help="comma list of plain, batched (the plain workload always runs: it gives `value`)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB 5 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -213,10 +339,13 @@ def main():
         out["bit_exact_vs_oracle"] = bool(np.array_equal(gstream, ostream))
         if not out["bit_exact_vs_oracle"]:
             raise RuntimeError("GPU codestream differs from the oracle")
-    print(json.dumps(out), flush=True)
     for b in (d_in, d_cs, d_out, d_ncw, d_status):
         b.free()
     lut.free()
+    wl = args.workloads.split(",")
+    if "batched" in wl:
+        out["batched"] = bench_batched(args, check=not args.no_cpu)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":
