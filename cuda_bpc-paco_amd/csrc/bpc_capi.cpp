@@ -94,6 +94,31 @@ int default_status(uint32_t** out) {
   return BPC_OK;
 }
 
+int device_cus(int dev) {
+  static std::mutex mu;
+  static std::vector<int> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  if ((int)cache.size() <= dev) cache.resize(dev + 1, 0);
+  if (!cache[dev]) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 1;
+    cache[dev] = cus;
+  }
+  return cache[dev];
+}
+
+// Bitplane scratch of the per-image kernel, one per device.  Launches that use it
+// are ordered through `last`: a launch on another stream first waits for the
+// previous user.
+struct DevScratch {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  hipEvent_t last = nullptr;
+  bool has_last = false;
+};
+std::mutex g_scratch_mu;
+std::vector<DevScratch> g_scratch;
+
 int launch(int decode, const int32_t* src, int32_t* dst, int size_x, int size_y, const int32_t* d_lut, int lut_len,
            const bpc_params* p, int fill, int32_t* d_ncw, uint32_t* d_status, hipStream_t stream, int maxp) {
   bpc::KArgs a;
@@ -113,8 +138,33 @@ int launch(int decode, const int32_t* src, int32_t* dst, int size_x, int size_y,
   a.lut_len = lut_len;
   a.layout = p->layout;
   a.fill = fill;
-  hipError_t e = bpc::launch_codec(decode, p->cpass3, maxp, a, stream);
+  if (a.ncb <= 0) return BPC_OK;
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  int per_cu = bpc::codec_resident_waves_per_cu(decode, p->cpass3, maxp);
+  if (per_cu <= 0) per_cu = 1;
+  const long long pairs = (a.ncb + 1) / 2;
+  const int grid = (int)std::min<long long>(pairs, (long long)per_cu * device_cus(dev));
+  const size_t need = (size_t)grid * bpc::scratch_bytes_per_wave(maxp);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch.size() <= dev) g_scratch.resize(dev + 1);
+  DevScratch& sc = g_scratch[dev];
+  if (!sc.last) HIP_TRY(hipEventCreateWithFlags(&sc.last, hipEventDisableTiming));
+  if (need > sc.bytes) {
+    if (sc.has_last) HIP_TRY(hipEventSynchronize(sc.last));
+    if (sc.ptr) HIP_TRY(hipFree(sc.ptr));
+    sc.ptr = nullptr;
+    sc.bytes = 0;
+    HIP_TRY(hipMalloc(&sc.ptr, need));
+    sc.bytes = need;
+  } else if (sc.has_last) {
+    HIP_TRY(hipStreamWaitEvent(stream, sc.last, 0));
+  }
+  a.scratch = sc.ptr;
+  hipError_t e = bpc::launch_codec(decode, p->cpass3, maxp, a, grid, stream);
   if (e != hipSuccess) return hip_fail(e, "kernel launch");
+  HIP_TRY(hipEventRecord(sc.last, stream));
+  sc.has_last = true;
   return BPC_OK;
 }
 
@@ -645,6 +695,10 @@ struct bpc_batch {
   uint32_t* d_ctl = nullptr;
   hipEvent_t staged = nullptr;  // the last descriptor upload has been consumed
   bool staged_pending = false;
+  hipEvent_t done = nullptr;  // the last run has finished
+  bool done_pending = false;
+  void* d_scratch = nullptr;  // bitplane scratch, one slot per resident wave
+  size_t scratch_bytes = 0;
   int waves_per_block = 4;
   int waves_per_simd = 0;  // 0: as many as fit
   int cus = 0;
@@ -657,12 +711,14 @@ namespace {
 void batch_release(bpc_batch* b) {
   for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
   b->graphs.clear();
-  void* dev[] = {b->d_images, b->d_offsets, b->d_keys, b->d_sorted, b->d_pair_keys, b->d_pair_queue, b->d_ctl};
+  void* dev[] = {b->d_images, b->d_offsets, b->d_keys,  b->d_sorted,
+                 b->d_pair_keys, b->d_pair_queue, b->d_ctl, b->d_scratch};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   if (b->h_images) (void)hipHostFree(b->h_images);
   if (b->h_offsets) (void)hipHostFree(b->h_offsets);
   if (b->staged) (void)hipEventDestroy(b->staged);
+  if (b->done) (void)hipEventDestroy(b->done);
 }
 
 struct BatchPlan {
@@ -725,10 +781,28 @@ int batch_stage(bpc_batch* b, int direction, const bpc_image* images, int n, Bat
   return BPC_OK;
 }
 
-int batch_enqueue(bpc_batch* b, int direction, int n, const BatchPlan& plan, int maxp, uint32_t* d_status,
-                  hipStream_t stream) {
+int batch_grid_blocks(bpc_batch* b, int direction, int maxp, const BatchPlan& plan) {
+  if (plan.total_pairs <= 0) return 0;
+  const int wpb = b->waves_per_block;
+  int per_cu = bpc::batch_resident_blocks_per_cu(direction == BPC_DECODE, maxp, wpb);
+  if (per_cu <= 0) per_cu = 1;
+  if (b->waves_per_simd > 0) per_cu = std::max(1, std::min(per_cu, (4 * b->waves_per_simd) / wpb));
+  long long blocks = (long long)per_cu * b->cus;
+  blocks = std::min(blocks, (plan.total_pairs + wpb - 1) / wpb);
+  return (int)blocks;
+}
+
+int batch_upload(bpc_batch* b, int n, hipStream_t stream) {
   HIP_TRY(hipMemcpyAsync(b->d_images, b->h_images, sizeof(bpc::BatchImage) * n, hipMemcpyHostToDevice, stream));
   HIP_TRY(hipMemcpyAsync(b->d_offsets, b->h_offsets, sizeof(long long) * (2 * n + 1), hipMemcpyHostToDevice, stream));
+  HIP_TRY(hipEventRecord(b->staged, stream));
+  b->staged_pending = true;
+  return BPC_OK;
+}
+
+// Kernel sequence of one batch run (captured into a graph when asked).
+int batch_kernels(bpc_batch* b, int direction, int n, const BatchPlan& plan, int maxp, int blocks,
+                  uint32_t* d_status, hipStream_t stream) {
   bpc::QueueArgs q;
   q.images = b->d_images;
   q.cb_offset = b->d_offsets;
@@ -746,20 +820,15 @@ int batch_enqueue(bpc_batch* b, int direction, int n, const BatchPlan& plan, int
   hipError_t e = bpc::launch_queue_build(q, stream);
   if (e != hipSuccess) return hip_fail(e, "queue build");
   bpc::BatchArgs a;
+  a.scratch = b->d_scratch;
   a.images = b->d_images;
   a.sorted = b->d_sorted;
   a.pair_queue = b->d_pair_queue;
   a.ctl = b->d_ctl;
   a.n_pairs = plan.total_pairs;
   a.rows = plan.rows;
-  if (plan.total_pairs > 0) {
-    const int wpb = b->waves_per_block;
-    int per_cu = bpc::batch_resident_blocks_per_cu(direction == BPC_DECODE, maxp, wpb);
-    if (per_cu <= 0) per_cu = 1;
-    if (b->waves_per_simd > 0) per_cu = std::max(1, std::min(per_cu, (4 * b->waves_per_simd) / wpb));
-    long long blocks = (long long)per_cu * b->cus;
-    blocks = std::min(blocks, (plan.total_pairs + wpb - 1) / wpb);
-    e = bpc::launch_batch(direction == BPC_DECODE, maxp, a, (int)blocks, wpb, stream);
+  if (blocks > 0) {
+    e = bpc::launch_batch(direction == BPC_DECODE, maxp, a, blocks, b->waves_per_block, stream);
     if (e != hipSuccess) return hip_fail(e, "batched kernel launch");
   }
   e = bpc::launch_status_fold(b->d_ctl, d_status, stream);
@@ -792,6 +861,7 @@ int bpc_batch_create(bpc_batch** out, int max_images, long long max_codeblocks) 
   if (e == hipSuccess) e = hipMalloc(&b->d_pair_queue, sizeof(uint32_t) * (entries / 2 + 1));
   if (e == hipSuccess) e = hipMalloc(&b->d_ctl, sizeof(uint32_t) * (bpc::CTL_IMG + 32 * (size_t)max_images));
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->staged, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
   if (e != hipSuccess) {
     batch_release(b);
     delete b;
@@ -803,7 +873,7 @@ int bpc_batch_create(bpc_batch** out, int max_images, long long max_codeblocks) 
 
 int bpc_batch_destroy(bpc_batch* b) {
   if (!b) return BPC_OK;
-  if (b->staged_pending) (void)hipEventSynchronize(b->staged);
+  if (b->done_pending) (void)hipEventSynchronize(b->done);
   batch_release(b);
   delete b;
   return BPC_OK;
@@ -834,25 +904,39 @@ int bpc_batch_run(bpc_batch* b, int direction, const bpc_image* images, int n_im
   if (st) return st;
   hipStream_t s = (hipStream_t)stream;
   const int maxp = plan.maxp;
+  const int blocks = batch_grid_blocks(b, direction, maxp, plan);
+  const size_t need = (size_t)blocks * b->waves_per_block * bpc::scratch_bytes_per_wave(maxp);
+  if (need > b->scratch_bytes) {  // grow the scratch once the previous run is done
+    if (b->done_pending) HIP_TRY(hipEventSynchronize(b->done));
+    for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+    b->graphs.clear();
+    if (b->d_scratch) HIP_TRY(hipFree(b->d_scratch));
+    b->d_scratch = nullptr;
+    b->scratch_bytes = 0;
+    HIP_TRY(hipMalloc(&b->d_scratch, need));
+    b->scratch_bytes = need;
+  }
+  st = batch_upload(b, n_images, s);
+  if (st) return st;
   if (!use_graph) {
-    st = batch_enqueue(b, direction, n_images, plan, maxp, d_status, s);
+    st = batch_kernels(b, direction, n_images, plan, maxp, blocks, d_status, s);
   } else {
-    char tun[64];
-    snprintf(tun, sizeof tun, "|%d,%d,%d,%p", b->waves_per_block, b->waves_per_simd, maxp, (void*)d_status);
+    char tun[96];
+    snprintf(tun, sizeof tun, "|%d,%d,%d,%d,%p", b->waves_per_block, b->waves_per_simd, maxp, blocks, (void*)d_status);
     const std::string key = plan.signature + tun;
     auto it = b->graphs.find(key);
     if (it == b->graphs.end()) {
-      hipStream_t cap = s;
-      bool own = false;
-      if (!cap) {  // the null stream cannot be captured
-        HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-        own = true;
+      hipStream_t cap = nullptr;
+      HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+      hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+      if (e != hipSuccess) {
+        (void)hipStreamDestroy(cap);
+        return hip_fail(e, "graph capture");
       }
-      HIP_TRY(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-      st = batch_enqueue(b, direction, n_images, plan, maxp, d_status, cap);
+      st = batch_kernels(b, direction, n_images, plan, maxp, blocks, d_status, cap);
       hipGraph_t graph = nullptr;
-      hipError_t e = hipStreamEndCapture(cap, &graph);
-      if (own) (void)hipStreamDestroy(cap);
+      e = hipStreamEndCapture(cap, &graph);
+      (void)hipStreamDestroy(cap);
       if (st) {
         if (graph) (void)hipGraphDestroy(graph);
         return st;
@@ -867,8 +951,8 @@ int bpc_batch_run(bpc_batch* b, int direction, const bpc_image* images, int n_im
     HIP_TRY(hipGraphLaunch(it->second, s));
   }
   if (st) return st;
-  HIP_TRY(hipEventRecord(b->staged, s));
-  b->staged_pending = true;
+  HIP_TRY(hipEventRecord(b->done, s));
+  b->done_pending = true;
   return BPC_OK;
 }
 

@@ -30,6 +30,7 @@ struct KArgs {
   int lut_len;         // ints in lut
   int layout;          // 0: raster (reference), 1: codeblock-contiguous (tile-major)
   int fill;            // encoder: write -1 to unused region words
+  void* scratch;       // per wave of the grid: 2 * MAXP * 64 uint2 (scratch_bytes_per_wave)
 };
 
 // One image of a batch, as the device sees it.
@@ -66,6 +67,7 @@ struct QueueArgs {
 };
 
 struct BatchArgs {
+  void* scratch;  // per resident wave: scratch_bytes_per_wave(maxp)
   const BatchImage* images;
   const uint32_t* sorted;
   const uint32_t* pair_queue;
@@ -76,7 +78,11 @@ struct BatchArgs {
 
 // Kernel variants hold this many bitplanes in registers.
 int planes_for(int nbp);
-hipError_t launch_codec(int decode, int cpass3, int maxp, const KArgs& a, hipStream_t stream);
+// Bitplane scratch of one wave: 2 row halves x MAXP planes x 64 lanes x 8 bytes.
+inline size_t scratch_bytes_per_wave(int maxp) { return (size_t)2 * maxp * 64 * 8; }
+// Waves of the per-image kernel that fit on one CU (occupancy query).
+int codec_resident_waves_per_cu(int decode, int cpass3, int maxp);
+hipError_t launch_codec(int decode, int cpass3, int maxp, const KArgs& a, int grid_waves, hipStream_t stream);
 hipError_t launch_queue_build(const QueueArgs& a, hipStream_t stream);
 hipError_t launch_batch(int decode, int maxp, const BatchArgs& a, int grid_blocks, int waves_per_block,
                         hipStream_t stream);

@@ -63,7 +63,6 @@ struct PlaneLut {  // probabilities of one bitplane, 7 bits each
 template <int MAXP>
 struct LaneState {
   u64 S[2], R[2], C[2], G[2];  // row masks per column (0 even, 1 odd): significant, refine, cleanup, sign
-  u64 P[MAXP][2];              // encoder: magnitude planes ; decoder: decoded planes
   u64 cur[2];                  // decoder: bits decoded in the current plane
   u32 L, Sz, slot, cw;         // coder: interval lower, interval size, reserved slot, codeword
   u32 cnt0, cnt1;              // next free slot of each half (wave-uniform)
@@ -296,9 +295,40 @@ struct HalfDesc {
 
 // Codes the two codeblocks of one wave (Kernel_BPC_CODER / Kernel_BPC_DECODER
 // per warp, BPC_PaCo.cu:1097-1224).
+// The wave's bitplane scratch, addressed through a buffer resource: the plane
+// offset goes in the scalar offset and the lane offset (lane * 8) in the vector
+// offset, so no per-plane 64-bit addresses occupy VGPRs.  Plane p of rows 0-31 is
+// at byte (p * 64) * 8, rows 32-63 at ((MAXP + p) * 64) * 8; one uint2 (even
+// column, odd column) per lane.  Accesses outside the slot are range-checked.
+constexpr int BUFFER_RSRC_WORD3 = 0x00020000;  // raw buffer, gfx9 family
+
+template <int MAXP>
+struct Scratch {
+  __amdgpu_buffer_rsrc_t rsrc;
+  u32 voff;
+  __device__ __forceinline__ Scratch(void* base, int lane)
+      : rsrc(__builtin_amdgcn_make_buffer_rsrc(base, (short)0, 2 * MAXP * 64 * 8, BUFFER_RSRC_WORD3)),
+        voff((u32)lane * 8u) {}
+  __device__ __forceinline__ void store(int slot, u32 even, u32 odd) const {
+    __attribute__((ext_vector_type(2))) u32 v = {even, odd};
+    __builtin_amdgcn_raw_buffer_store_b64(v, rsrc, voff, slot * 512, 0);
+  }
+  __device__ __forceinline__ uint2 load(int slot) const {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(rsrc, voff, slot * 512, 0);
+    return make_uint2(v[0], v[1]);
+  }
+  // magnitude plane bp of both columns, all rows
+  __device__ __forceinline__ void plane(int bp, u64* out) const {
+    const uint2 lo = load(bp), hi = load(MAXP + bp);
+    out[0] = (u64)lo.x | ((u64)hi.x << 32);
+    out[1] = (u64)lo.y | ((u64)hi.y << 32);
+  }
+};
+
 template <int DEC, int CP3, int MAXP>
-__device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uint32_t* status_word) {
+__device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uint32_t* status_word, void* scr_base) {
   const int lane = threadIdx.x & 63;
+  const Scratch<MAXP> scr(scr_base, lane);
   const int half = lane >> 5;
   const int hl = lane & 31;
   const int cb = a.cb;
@@ -325,11 +355,7 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
 
   LaneState<MAXP> s;
 #pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    s.S[c] = s.R[c] = s.C[c] = s.G[c] = s.cur[c] = 0;
-#pragma unroll
-    for (int p = 0; p < MAXP; ++p) s.P[p][c] = 0;
-  }
+  for (int c = 0; c < 2; ++c) s.S[c] = s.R[c] = s.C[c] = s.G[c] = s.cur[c] = 0;
   s.L = 0;
   s.Sz = 0;
   s.slot = 0;
@@ -372,11 +398,9 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
       }
       s.G[0] |= (u64)glo0 << (32 * h);
       s.G[1] |= (u64)glo1 << (32 * h);
+      // plane p, rows of this half, both columns -> the wave's scratch
 #pragma unroll
-      for (int p = 0; p < MAXP; ++p) {
-        s.P[p][0] |= (u64)plo[p][0] << (32 * h);
-        s.P[p][1] |= (u64)plo[p][1] << (32 * h);
-      }
+      for (int p = 0; p < MAXP; ++p) scr.store(h * MAXP + p, plo[p][0], plo[p][1]);
     }
     // Find_MSB (BPC_PaCo.cu:114-161): OR over the 32 lanes of the half.
 #pragma unroll
@@ -425,17 +449,9 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
   const int msb_eff = good ? msb : -1;
   const int top = max(__builtin_amdgcn_readlane(msb_eff, 0), __builtin_amdgcn_readlane(msb_eff, 32));
 
-  if (!DEC) {
-    // align the shift register so that P[MAXP-1] holds plane `top`
-    for (int k = top; k < MAXP - 1; ++k) {
-#pragma unroll
-      for (int p = MAXP - 1; p > 0; --p) {
-        s.P[p][0] = s.P[p - 1][0];
-        s.P[p][1] = s.P[p - 1][1];
-      }
-      s.P[0][0] = s.P[0][1] = 0;
-    }
-  }
+  // encoder: magnitude plane `top`, and the next one prefetched during each plane
+  u64 plane[2] = {0, 0};
+  if (!DEC && top >= 0) scr.plane(top, plane);
 
   // Bitplane driver (Encode / Decode, BPC_PaCo.cu:968-1090).  In 3-pass mode the
   // reference's opening cleanup pass at MSB equals plane MSB with SPP and MRP
@@ -448,11 +464,8 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
     pl.sign = pl.cpsign = pl.ref = 0;
     if (act) pl = load_plane_lut<CP3>(a.lut, ref0, sig0, sign0, cpoff, bp, lbad);
     if (lbad) status |= ST_RANGE;
-    u64 plane[2] = {0, 0};
-    if (!DEC) {
-      plane[0] = s.P[MAXP - 1][0];
-      plane[1] = s.P[MAXP - 1][1];
-    }
+    u64 next[2] = {0, 0};
+    if (!DEC && bp > 0) scr.plane(bp - 1, next);
     const bool act_sr = act && !(CP3 && bp == msb);
     if (__ballot(act_sr)) {
       sig_pass<DEC, CP3, 0>(s, act_sr, plane, pl, g, a.src, a.dst);
@@ -460,20 +473,11 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
     }
     if (CP3) sig_pass<DEC, CP3, 1>(s, act, plane, pl, g, a.src, a.dst);
     if (!DEC) {
-#pragma unroll
-      for (int p = MAXP - 1; p > 0; --p) {
-        s.P[p][0] = s.P[p - 1][0];
-        s.P[p][1] = s.P[p - 1][1];
-      }
-      s.P[0][0] = s.P[0][1] = 0;
-    } else {
-#pragma unroll
-      for (int p = 0; p < MAXP - 1; ++p) {
-        s.P[p][0] = s.P[p + 1][0];
-        s.P[p][1] = s.P[p + 1][1];
-      }
-      s.P[MAXP - 1][0] = s.cur[0];
-      s.P[MAXP - 1][1] = s.cur[1];
+      plane[0] = next[0];
+      plane[1] = next[1];
+    } else {  // decoded magnitude bits of plane bp -> the wave's scratch
+      scr.store(bp, (u32)s.cur[0], (u32)s.cur[1]);
+      scr.store(MAXP + bp, (u32)(s.cur[0] >> 32), (u32)(s.cur[1] >> 32));
       s.cur[0] = s.cur[1] = 0;
     }
   }
@@ -493,18 +497,28 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
     // Write_Coefficients (BPC_PaCo.cu:46-63): magnitude from the decoded planes
     const bool emit = good && !overflow;
     if (valid) {
-      for (int r = 0; r < rows; ++r) {
-        u32 m0 = 0, m1 = 0;
 #pragma unroll
-        for (int p = 0; p < MAXP; ++p) {
-          m0 |= ((u32)(s.P[MAXP - 1 - p][0] >> r) & 1u) << p;
-          m1 |= ((u32)(s.P[MAXP - 1 - p][1] >> r) & 1u) << p;
+      for (int h = 0; h < 2; ++h) {
+        const int r0 = 32 * h, r1 = min(rows, 32 * h + 32);
+        if (r0 >= r1) break;
+        uint2 pw[MAXP];  // planes 0..top of this half's rows; unwritten planes read as 0
+#pragma unroll
+        for (int p = 0; p < MAXP; ++p) pw[p] = p <= top ? scr.load(h * MAXP + p) : make_uint2(0u, 0u);
+        const u32 g0 = (u32)(s.G[0] >> (32 * h)), g1 = (u32)(s.G[1] >> (32 * h));
+        for (int r = r0; r < r1; ++r) {
+          const int sh = r - r0;
+          u32 m0 = 0, m1 = 0;
+#pragma unroll
+          for (int p = 0; p < MAXP; ++p) {
+            m0 |= ((pw[p].x >> sh) & 1u) << p;
+            m1 |= ((pw[p].y >> sh) & 1u) << p;
+          }
+          int v0 = (int)m0, v1 = (int)m1;
+          if ((g0 >> sh) & 1u) v0 = -v0;
+          if ((g1 >> sh) & 1u) v1 = -v1;
+          if (!emit) v0 = v1 = 0;
+          *reinterpret_cast<int2*>(a.dst + cbase + (int64_t)r * rs) = make_int2(v0, v1);
         }
-        int v0 = (int)m0, v1 = (int)m1;
-        if ((s.G[0] >> r) & 1u) v0 = -v0;
-        if ((s.G[1] >> r) & 1u) v1 = -v1;
-        if (!emit) v0 = v1 = 0;
-        *reinterpret_cast<int2*>(a.dst + cbase + (int64_t)r * rs) = make_int2(v0, v1);
       }
     }
   }
@@ -514,8 +528,9 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
 
 template <int DEC, int CP3, int MAXP>
 __global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
-  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  const int cb = 2 * wave + ((threadIdx.x & 63) >> 5);
+  const int wave = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
+  void* scr = reinterpret_cast<char*>(a.scratch) + (size_t)wave * (2 * MAXP * 64 * 8);
   HalfDesc d;
   d.src = a.src;
   d.dst = a.dst;
@@ -530,8 +545,12 @@ __global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
   d.lut_len = a.lut_len;
   d.layout = a.layout;
   d.fill = a.fill;
-  d.cb = cb < a.ncb ? cb : -1;
-  code_pair<DEC, CP3, MAXP>(d, a.rows, a.status);
+  const int npairs = (a.ncb + 1) / 2;
+  for (int pair = wave; pair < npairs; pair += nwaves) {
+    const int cb = 2 * pair + ((threadIdx.x & 63) >> 5);
+    d.cb = cb < a.ncb ? cb : -1;
+    code_pair<DEC, CP3, MAXP>(d, a.rows, a.status, scr);
+  }
 }
 
 // ---- batched path ---------------------------------------------------------
@@ -547,6 +566,8 @@ __device__ __forceinline__ u32 msb_bucket(int msb) { return (msb >= 0 && msb <= 
 template <int DEC, int MAXP>
 __global__ __launch_bounds__(256) void bpc_batch_kernel(const BatchArgs a) {
   const int half = (threadIdx.x & 63) >> 5;
+  const size_t wave = (size_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  void* scr = reinterpret_cast<char*>(a.scratch) + wave * (2 * MAXP * 64 * 8);
   for (;;) {
     u32 slot = 0;
     if ((threadIdx.x & 63) == 0) slot = atomicAdd(&a.ctl[CTL_HEAD], 1u);
@@ -572,9 +593,9 @@ __global__ __launch_bounds__(256) void bpc_batch_kernel(const BatchArgs a) {
     d.fill = im.fill;
     d.cb = e == EMPTY ? -1 : (int)(e & 0xFFFFFu);
     if (im.cpass3)
-      code_pair<DEC, 1, MAXP>(d, a.rows, &a.ctl[CTL_STATUS]);
+      code_pair<DEC, 1, MAXP>(d, a.rows, &a.ctl[CTL_STATUS], scr);
     else
-      code_pair<DEC, 0, MAXP>(d, a.rows, &a.ctl[CTL_STATUS]);
+      code_pair<DEC, 0, MAXP>(d, a.rows, &a.ctl[CTL_STATUS], scr);
   }
 }
 
@@ -703,20 +724,42 @@ int planes_for(int nbp) {
 }
 
 template <int DEC, int CP3, int MAXP>
-static hipError_t launch_one(const KArgs& a, hipStream_t stream) {
-  const int waves = (a.ncb + 1) / 2;
-  hipLaunchKernelGGL((bpc_codec_kernel<DEC, CP3, MAXP>), dim3(waves), dim3(64), 0, stream, a);
+static hipError_t launch_one(const KArgs& a, int grid_waves, hipStream_t stream) {
+  hipLaunchKernelGGL((bpc_codec_kernel<DEC, CP3, MAXP>), dim3(grid_waves), dim3(64), 0, stream, a);
   return hipGetLastError();
 }
 
+template <int DEC, int CP3, int MAXP>
+static int plain_resident() {
+  int blocks = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, bpc_codec_kernel<DEC, CP3, MAXP>, 64, 0) != hipSuccess)
+    return 0;
+  return blocks;
+}
+
 template <int DEC, int CP3>
-static hipError_t launch_p(int maxp, const KArgs& a, hipStream_t stream) {
+static hipError_t launch_p(int maxp, const KArgs& a, int grid_waves, hipStream_t stream) {
   switch (maxp) {
-    case 16: return launch_one<DEC, CP3, 16>(a, stream);
-    case 20: return launch_one<DEC, CP3, 20>(a, stream);
-    case 28: return launch_one<DEC, CP3, 28>(a, stream);
+    case 16: return launch_one<DEC, CP3, 16>(a, grid_waves, stream);
+    case 20: return launch_one<DEC, CP3, 20>(a, grid_waves, stream);
+    case 28: return launch_one<DEC, CP3, 28>(a, grid_waves, stream);
     default: return hipErrorInvalidValue;
   }
+}
+
+template <int DEC, int CP3>
+static int resident_p(int maxp) {
+  switch (maxp) {
+    case 16: return plain_resident<DEC, CP3, 16>();
+    case 20: return plain_resident<DEC, CP3, 20>();
+    case 28: return plain_resident<DEC, CP3, 28>();
+    default: return 0;
+  }
+}
+
+int codec_resident_waves_per_cu(int decode, int cpass3, int maxp) {
+  if (decode) return cpass3 ? resident_p<1, 1>(maxp) : resident_p<1, 0>(maxp);
+  return cpass3 ? resident_p<0, 1>(maxp) : resident_p<0, 0>(maxp);
 }
 
 template <int DEC, int MAXP>
@@ -780,10 +823,10 @@ hipError_t launch_status_fold(uint32_t* ctl, uint32_t* status, hipStream_t strea
   return hipGetLastError();
 }
 
-hipError_t launch_codec(int decode, int cpass3, int maxp, const KArgs& a, hipStream_t stream) {
+hipError_t launch_codec(int decode, int cpass3, int maxp, const KArgs& a, int grid_waves, hipStream_t stream) {
   if (a.ncb <= 0) return hipSuccess;
-  if (decode) return cpass3 ? launch_p<1, 1>(maxp, a, stream) : launch_p<1, 0>(maxp, a, stream);
-  return cpass3 ? launch_p<0, 1>(maxp, a, stream) : launch_p<0, 0>(maxp, a, stream);
+  if (decode) return cpass3 ? launch_p<1, 1>(maxp, a, grid_waves, stream) : launch_p<1, 0>(maxp, a, grid_waves, stream);
+  return cpass3 ? launch_p<0, 1>(maxp, a, grid_waves, stream) : launch_p<0, 0>(maxp, a, grid_waves, stream);
 }
 
 }  // namespace bpc
