@@ -68,6 +68,9 @@ struct LaneState {
   u32 cnt0, cnt1;              // next free slot of each half (wave-uniform)
 };
 
+// Lane mask of a predicate, taken straight from the compare (no 0/1 round trip).
+__device__ __forceinline__ u64 ballot(bool b) { return __builtin_amdgcn_ballot_w64(b); }
+
 __device__ __forceinline__ u32 dpp_from_left(u32 v) {  // lane l receives lane l-1
   return (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xF, 0xF, true);  // wave_shr:1
 }
@@ -90,6 +93,8 @@ struct Geo {
   int64_t region;  // first word of the codeblock's region
   int stride;
   int rows;
+  u32 edge_even;   // 7, or 0 for the first lane of a half (no left neighbour in the codeblock)
+  u32 edge_odd;    // 7, or 0 for the last lane of a half (no right neighbour)
 };
 
 // Slot reservation for one coder call site (BPC_PaCo.cu:268-277 / :304-316):
@@ -98,17 +103,25 @@ struct Geo {
 // counters stay wave-uniform (SGPRs).
 template <int DEC, int MAXP>
 __device__ __forceinline__ void reserve(LaneState<MAXP>& s, bool at, const Geo& g, const int32_t* src) {
-  const bool res = at && s.Sz == 0;
-  const u64 m = __ballot(res);
+  // 0 exactly for the lanes that reserve: one compare feeds the ballot directly
+  const u32 szr = at ? s.Sz : 1u;
+  const u64 m = ballot(szr == 0);
   if (m) {
     const u32 lo = (u32)m, hi = (u32)(m >> 32);
-    const u32 rank = __builtin_amdgcn_mbcnt_hi(g.half ? hi : 0u, __builtin_amdgcn_mbcnt_lo(g.half ? 0u : lo, 0u));
-    const u32 slot = (g.half ? s.cnt1 : s.cnt0) + rank;
-    s.slot = res ? slot : s.slot;
-    s.L = res ? 0u : s.L;
-    s.Sz = res ? 0xFFFFu : s.Sz;
-    if (DEC && res) s.cw = slot < (u32)g.stride ? (u32)src[g.region + slot] : 0u;
-    s.cnt0 = __builtin_amdgcn_readfirstlane(s.cnt0 + __builtin_popcount(lo));
+    const u32 nlo = __builtin_popcount(lo);
+    // mbcnt over the whole wave counts, for an upper-half lane, every lower-half
+    // lane too; its base is lowered by that count
+    const u32 base = g.half ? s.cnt1 - nlo : s.cnt0;
+    const u32 slot = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, base));
+    if (szr == 0) {
+      // A slot past the region (overflow, reported from the final counter) is
+      // redirected to the region's last word so that no access leaves the region.
+      s.slot = min(slot, (u32)g.stride - 1u);
+      s.L = 0;
+      s.Sz = 0xFFFFu;
+      if (DEC) s.cw = (u32)src[g.region + s.slot];
+    }
+    s.cnt0 = __builtin_amdgcn_readfirstlane(s.cnt0 + nlo);
     s.cnt1 = __builtin_amdgcn_readfirstlane(s.cnt1 + __builtin_popcount(hi));
   }
 }
@@ -123,7 +136,7 @@ __device__ __forceinline__ void enc_symbol(LaneState<MAXP>& s, u32 sym, u32 prob
   } else {
     s.Sz = aux;
   }
-  if (s.Sz == 0 && s.slot < (u32)g.stride) dst[g.region + s.slot] = (int32_t)s.L;
+  if (s.Sz == 0) dst[g.region + s.slot] = (int32_t)s.L;
 }
 
 // Decoder coder step (BPC_PaCo.cu:321-337).
@@ -150,17 +163,18 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
   const u64 bit = 1ull << i;
   // 3-row windows (rows i-1, i, i+1 at bits 0..2); rows -1 and past the end are 0.
   const u32 wo = FIRST ? ((u32)(s.S[o] << 1) & 7u) : ((u32)(s.S[o] >> (i - 1)) & 7u);
-  const u32 wc = FIRST ? ((u32)(s.S[c] << 1) & 5u) : ((u32)(s.S[c] >> (i - 1)) & 5u);
+  const u32 wc = FIRST ? ((u32)(s.S[c] << 1) & 7u) : ((u32)(s.S[c] >> (i - 1)) & 7u);
   // the neighbour lane's column (left lane's odd column for an even step, right
-  // lane's even column for an odd step; BPC_PaCo.cu:342-388) with its sign bit at row i
-  const u32 pk = wo | (((u32)(s.G[o] >> i) & 1u) << 3);
-  u32 nb = ODD ? dpp_from_right(pk) : dpp_from_left(pk);
-  if (g.hl == (ODD ? 31 : 0)) nb = 0;  // Correct_CB_borders, BPC_PaCo.cu:390-410
-  const u32 ctx = __builtin_popcount(wo) + __builtin_popcount(wc) + __builtin_popcount(nb & 7u);
+  // lane's even column for an odd step; BPC_PaCo.cu:342-388); lanes at the
+  // codeblock edge see zeros (Correct_CB_borders, BPC_PaCo.cu:390-410)
+  const u32 nb = (ODD ? dpp_from_right(wo) : dpp_from_left(wo)) & (ODD ? g.edge_odd : g.edge_even);
+  // context = significant neighbours; wc includes the coefficient itself, which is
+  // not significant wherever a context is used
+  const u32 ctx = __builtin_popcount(wo) + __builtin_popcount(wc) + __builtin_popcount(nb);
 
   bool at;
   if (!CLEAN) {
-    const bool cand = act && !((s.S[c] >> i) & 1u);
+    const bool cand = act && !(wc & 2u);
     if (CP3) {
       const bool gate = ctx != 0;  // BPC_PaCo.cu:424
       if (cand && !gate) s.C[c] |= bit;
@@ -183,22 +197,24 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
     } else {
       sym = dec_symbol(s, prob);
     }
-  }
-  const bool nsig = at && sym;
-  if (nsig) {
-    s.S[c] |= bit;
-    if (CLEAN) s.R[c] |= bit;  // BPC_PaCo.cu:534
-    if (DEC) s.cur[c] |= bit;  // magnitude bit i of this plane
+    const u64 nb64 = (u64)sym << i;
+    s.S[c] |= nb64;
+    if (CLEAN) s.R[c] |= nb64;  // BPC_PaCo.cu:534
+    if (DEC) s.cur[c] |= nb64;  // magnitude bit i of this plane
   }
   // site 2: sign symbol of the coefficients that became significant
-  if (__ballot(nsig)) {
+  if (ballot(sym != 0)) {
+    const bool nsig = sym != 0;
+    // neighbour lane's sign bit at row i, moved like its significance window
+    const u32 gs = (u32)(s.G[o] >> i) & 1u;
+    const u32 gn = (ODD ? dpp_from_right(gs) : dpp_from_left(gs)) & ((nb >> 1) & 1u);
     // (significant, negative) of the vertical pair U, D (own column rows i-1, i+1)
     // and of the horizontal pair: neighbour lane's column and own other column at row i
     const u32 gc = FIRST ? ((u32)(s.G[c] << 1) & 5u) : ((u32)(s.G[c] >> (i - 1)) & 5u);
-    const u32 vidx = wc | ((gc & wc) << 1);
-    const u32 so = (wo >> 1) & 1u, go = (u32)(s.G[o] >> i) & so;
-    const u32 sn = (nb >> 1) & 1u, gn = (nb >> 3) & sn;
-    const u32 hidx = sn | (gn << 1) | (so << 2) | (go << 3);
+    const u32 wv = wc & 5u;
+    const u32 vidx = wv | ((gc & wv) << 1);
+    const u32 so = (wo >> 1) & 1u, go = gs & so;
+    const u32 hidx = ((nb >> 1) & 1u) | (gn << 1) | (so << 2) | (go << 3);
     const u32 vs = (PAIR_SIGN_TABLE >> (2 * vidx)) & 3u;
     const u32 hs = (PAIR_SIGN_TABLE >> (2 * hidx)) & 3u;
     const u32 sctx = (u32)(SIGN_CTX_TABLE >> (4 * (3 * hs + vs))) & 0xFu;
@@ -234,23 +250,53 @@ __device__ __forceinline__ void ref_step(LaneState<MAXP>& s, int i, bool act, u6
   }
 }
 
+// OR of a 64-bit row mask over the wave, wave-uniform.
+__device__ __forceinline__ u64 wave_or(u64 x) {
+  u32 lo = (u32)x, hi = (u32)(x >> 32);
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    lo |= (u32)__shfl_xor((int)lo, off, 64);
+    hi |= (u32)__shfl_xor((int)hi, off, 64);
+  }
+  return ((u64)__builtin_amdgcn_readfirstlane(hi) << 32) | (u32)__builtin_amdgcn_readfirstlane(lo);
+}
+
+// Steps whose row holds no coefficient that can take part in the pass, in any lane
+// of the wave, change nothing and are skipped.  Row sets are taken at pass start
+// and are supersets of the rows that take part: SPP codes only coefficients that
+// are not significant (significance only grows, one coefficient per step); CP
+// only those with the cleanup flag (never set during CP); MRP only significant ones.
 template <int DEC, int CP3, int CLEAN, int MAXP>
 __device__ __forceinline__ void sig_pass(LaneState<MAXP>& s, bool act, const u64* plane, const PlaneLut& pl,
                                          const Geo& g, const int32_t* src, int32_t* dst) {
-  sig_step<0, true, DEC, CP3, CLEAN>(s, 0, act, plane[0], pl, g, src, dst);
-  sig_step<1, true, DEC, CP3, CLEAN>(s, 0, act, plane[1], pl, g, src, dst);
-  for (int i = 1; i < g.rows; ++i) {
-    sig_step<0, false, DEC, CP3, CLEAN>(s, i, act, plane[0], pl, g, src, dst);
-    sig_step<1, false, DEC, CP3, CLEAN>(s, i, act, plane[1], pl, g, src, dst);
+  const u64 rowmask = g.rows >= 64 ? ~0ull : ((1ull << g.rows) - 1ull);
+  const u64 mE = wave_or(act ? (CLEAN ? s.C[0] : (~s.S[0] & rowmask)) : 0ull);
+  const u64 mO = wave_or(act ? (CLEAN ? s.C[1] : (~s.S[1] & rowmask)) : 0ull);
+  u64 rows = mE | mO;
+  if (rows & 1ull) {
+    if (mE & 1ull) sig_step<0, true, DEC, CP3, CLEAN>(s, 0, act, plane[0], pl, g, src, dst);
+    if (mO & 1ull) sig_step<1, true, DEC, CP3, CLEAN>(s, 0, act, plane[1], pl, g, src, dst);
+  }
+  rows &= ~1ull;
+  while (rows) {
+    const int i = __builtin_ctzll(rows);
+    rows &= rows - 1ull;
+    if ((mE >> i) & 1ull) sig_step<0, false, DEC, CP3, CLEAN>(s, i, act, plane[0], pl, g, src, dst);
+    if ((mO >> i) & 1ull) sig_step<1, false, DEC, CP3, CLEAN>(s, i, act, plane[1], pl, g, src, dst);
   }
 }
 
 template <int DEC, int MAXP>
 __device__ __forceinline__ void ref_pass(LaneState<MAXP>& s, bool act, const u64* plane, const PlaneLut& pl,
                                          const Geo& g, const int32_t* src, int32_t* dst) {
-  for (int i = 0; i < g.rows; ++i) {
-    ref_step<0, DEC>(s, i, act, plane[0], pl, g, src, dst);
-    ref_step<1, DEC>(s, i, act, plane[1], pl, g, src, dst);
+  const u64 mE = wave_or(act ? s.S[0] : 0ull);
+  const u64 mO = wave_or(act ? s.S[1] : 0ull);
+  u64 rows = mE | mO;
+  while (rows) {
+    const int i = __builtin_ctzll(rows);
+    rows &= rows - 1ull;
+    if ((mE >> i) & 1ull) ref_step<0, DEC>(s, i, act, plane[0], pl, g, src, dst);
+    if ((mO >> i) & 1ull) ref_step<1, DEC>(s, i, act, plane[1], pl, g, src, dst);
   }
 }
 
@@ -343,6 +389,8 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
   g.region = (int64_t)cbc * a.stride;  // WCoordinate, BPC_PaCo.cu:1130
   g.stride = a.stride;
   g.rows = rows;
+  g.edge_even = hl == 0 ? 0u : 7u;
+  g.edge_odd = hl == 31 ? 0u : 7u;
   int64_t cbase;
   int rs;
   if (a.layout) {
@@ -441,9 +489,9 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
   }
   // a codeblock is coded only if every lane of its half is good
   {
-    const u64 ok = __ballot(good);
+    const u64 ok = ballot(good);
     const u64 hm = half ? 0xFFFFFFFF00000000ull : 0x00000000FFFFFFFFull;
-    const u64 vm = __ballot(valid);
+    const u64 vm = ballot(valid);
     good = good && ((ok & hm) == (vm & hm));
   }
   const int msb_eff = good ? msb : -1;
@@ -467,7 +515,7 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
     u64 next[2] = {0, 0};
     if (!DEC && bp > 0) scr.plane(bp - 1, next);
     const bool act_sr = act && !(CP3 && bp == msb);
-    if (__ballot(act_sr)) {
+    if (ballot(act_sr)) {
       sig_pass<DEC, CP3, 0>(s, act_sr, plane, pl, g, a.src, a.dst);
       ref_pass<DEC>(s, act_sr, plane, pl, g, a.src, a.dst);
     }
@@ -487,7 +535,7 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
   if (overflow) status |= ST_OVERFLOW;
   if (!DEC) {
     // flush of each lane's open codeword (BPC_PaCo.cu:1020)
-    if (good && s.slot < (u32)a.stride) a.dst[g.region + s.slot] = (int32_t)s.L;
+    if (good) a.dst[g.region + s.slot] = (int32_t)s.L;
     // -1 in every word past the used slots (BPC_PaCo.cu:1284); a codeblock that was
     // not coded (all zero, or rejected) keeps only its MSB word
     if (valid && a.fill)
