@@ -205,6 +205,121 @@ def bench_batched(args, check=True):
     return out
 
 
+def bench_served(args):
+    """Open-loop Poisson stream of 50% encode / 50% decode requests, sizes uniform
+    over {2048^2, 3072^2, 4096^2, 6144^2}, 14-16 bitplanes, 2 or 3 passes; decode
+    requests replay codestreams produced earlier in the run and kept in HBM.  The
+    continuous-batching server against a plain one-request-at-a-time loop."""
+    import random
+    import threading
+    sizes = [2048, 3072, 4096, 6144]
+    variants = [(0, 15, 14), (1, 16, 16)]  # (cpass3, LUT bitplanes, magnitude bitplanes)
+    pool = []
+    log("served: generating request pool")
+    for si, size in enumerate(sizes):
+        for vi, (cp3, nbp, mag) in enumerate(variants):
+            seed = args.seed + 5000 + 10 * si + 2 * vi
+            img = synth.synth_image(size, size, 5, seed, mag_bits=mag)
+            lut_arr = bpc.train_lut(synth.synth_image(size, size, 5, seed + 1, mag_bits=mag),
+                                    bpc.params(cpass3=cp3, lut_n_bitplanes=nbp))
+            p = bpc.params(cpass3=cp3, lut_n_bitplanes=nbp, layout=bpc.LAYOUT_TILED)
+            lut = bpc.Lut.from_array(lut_arr)
+            d_img = bpc.DeviceBuffer.from_array(synth.to_tiled(img))
+            d_cs = bpc.DeviceBuffer(bpc.stream_words(size, size, p))
+            bpc.encode_device(d_img, d_cs, size, size, lut, p)  # the codestream decode requests replay
+            pool.append(dict(size=size, p=p, lut=lut, d_img=d_img, d_cs=d_cs))
+    if bpc.status_take() != 0:
+        raise RuntimeError("served pool encode failed")
+    d_tmp_cs = bpc.DeviceBuffer(max(bpc.stream_words(it["size"], it["size"], it["p"]) for it in pool))
+    d_tmp_img = bpc.DeviceBuffer(max(it["size"] ** 2 for it in pool))
+    stream = bpc.Stream()
+
+    def make_plan(n, seed):
+        r = random.Random(seed)
+        return [(r.randrange(len(pool)), r.random() < 0.5) for _ in range(n)]
+
+    # plain loop: one request at a time, per-image launches
+    def plain_loop(plan):
+        t0 = time.perf_counter()
+        lat = []
+        for j, enc in plan:
+            it = pool[j]
+            t = time.perf_counter()
+            if enc:
+                bpc.encode_device(it["d_img"], d_tmp_cs, it["size"], it["size"], it["lut"], it["p"], stream=stream)
+            else:
+                bpc.decode_device(it["d_cs"], d_tmp_img, it["size"], it["size"], it["lut"], it["p"], stream=stream)
+            stream.sync()
+            lat.append((time.perf_counter() - t) * 1e3)
+        wall = time.perf_counter() - t0
+        return sum(pool[j]["size"] ** 2 for j, _ in plan) / wall / 1e6, lat
+
+    plain_loop(make_plan(8, 1))
+    n_req = 240
+    plain_rate, plain_lat = plain_loop(make_plan(n_req, 2))
+    mean_samples = float(np.mean([it["size"] ** 2 for it in pool]))
+    capacity_rps = plain_rate * 1e6 / mean_samples  # one-at-a-time service rate
+    srv = bpc.Server(max_inflight=3, max_batch_images=64, hbm_budget_bytes=24 << 30)
+    sweep = []
+    try:
+        for mult in (0.5, 1.0, 2.0, 3.0, 4.0):
+            rate = mult * capacity_rps
+            plan = make_plan(n_req, int(mult * 100))
+            rng = random.Random(int(mult * 1000))
+            gaps = [rng.expovariate(rate) for _ in plan]
+            ids = [None] * len(plan)
+            done_t = [None] * len(plan)
+            results = [None] * len(plan)
+
+            def waiter():
+                for k in range(len(plan)):
+                    while ids[k] is None:
+                        time.sleep(0.0002)
+                    results[k] = srv.wait(ids[k], timeout_ms=120000)
+                    done_t[k] = time.perf_counter()
+                    srv.release(ids[k])
+
+            th = threading.Thread(target=waiter)
+            th.start()
+            t0 = time.perf_counter()
+            t_next = t0
+            for k, (j, enc) in enumerate(plan):
+                t_next += gaps[k]
+                dt = t_next - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                it = pool[j]
+                ids[k] = srv.submit(bpc.CODE if enc else bpc.DECODE, it["d_img"] if enc else it["d_cs"], it["lut"],
+                                    it["size"], it["size"], it["p"])
+            th.join()
+            if any(r.status != 0 for r in results):
+                raise RuntimeError("served request failed")
+            wall = max(done_t) - t0
+            lat = [r.latency_us / 1e3 for r in results]
+            samples = sum(pool[j]["size"] ** 2 for j, _ in plan)
+            sweep.append({"offered_rps": round(rate, 1), "offered_msamples_s": round(rate * mean_samples / 1e6, 1),
+                          "sustained_msamples_s": round(samples / wall / 1e6, 1),
+                          "latency_ms_p50_p99": percentiles(lat), "requests": len(plan)})
+        st = srv.stats()
+        batches = int(st.batches)
+    finally:
+        srv.destroy()
+    best = max(sweep, key=lambda r: r["sustained_msamples_s"])
+    out = {"workload": "served (Poisson, 50% encode / 50% decode, sizes {2048,3072,4096,6144}^2, 14-16 bitplanes, "
+                       "2/3 passes)",
+           "plain_loop_msamples_s": round(plain_rate, 1), "plain_loop_latency_ms_p50_p99": percentiles(plain_lat),
+           "sweep": sweep, "best_sustained_msamples_s": best["sustained_msamples_s"],
+           "best_vs_plain_loop": round(best["sustained_msamples_s"] / plain_rate, 3),
+           "batches_launched": batches, "server": {"max_inflight": 3, "max_batch_images": 64}}
+    for it in pool:
+        it["d_img"].free()
+        it["d_cs"].free()
+        it["lut"].free()
+    d_tmp_cs.free()
+    d_tmp_img.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,8 +327,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle baseline and parity check")
-    ap.add_argument("--workloads", default="plain,batched",
-                    help="comma list of plain, batched (the plain workload always runs: it gives `value`)")
+    ap.add_argument("--workloads", default="plain,batched,served",
+                    help="comma list of plain, batched, served (the plain workload always runs: it gives `value`)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB 5 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -345,6 +460,8 @@ def main():
     wl = args.workloads.split(",")
     if "batched" in wl:
         out["batched"] = bench_batched(args, check=not args.no_cpu)
+    if "served" in wl:
+        out["served"] = bench_served(args)
     print(json.dumps(out), flush=True)
 
 

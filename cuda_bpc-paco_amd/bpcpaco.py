@@ -34,6 +34,7 @@ ERR_RANGE = -4
 ERR_MALFORMED = -5
 ERR_WIDE = -6
 ERR_NODEVICE = -7
+ERR_TIMEOUT = -8
 
 
 class BPCError(RuntimeError):
@@ -73,6 +74,28 @@ class Image(ctypes.Structure):
         ("params", Params),
         ("fill_unused", ctypes.c_int32),
     ]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("max_inflight", ctypes.c_int32), ("max_batch_images", ctypes.c_int32),
+                ("max_batch_codeblocks", ctypes.c_longlong), ("hbm_budget_bytes", ctypes.c_size_t),
+                ("waves_per_block", ctypes.c_int32)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("direction", ctypes.c_int32), ("src", ctypes.c_void_p), ("lut", ctypes.c_void_p),
+                ("lut_len", ctypes.c_int32), ("size_x", ctypes.c_int32), ("size_y", ctypes.c_int32),
+                ("params", Params)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("output", ctypes.c_void_p), ("ncw", ctypes.c_void_p),
+                ("queue_us", ctypes.c_double), ("latency_us", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("batches", ctypes.c_ulonglong), ("completed", ctypes.c_ulonglong), ("pending", ctypes.c_ulonglong),
+                ("committed_bytes", ctypes.c_size_t), ("max_batch_images", ctypes.c_size_t)]
 
 
 def params(cpass3=0, dwt_levels=5, cblock_length=64, lut_n_bitplanes=15, cb_stride_words=0, layout=LAYOUT_RASTER):
@@ -126,6 +149,13 @@ def lib():
             "bpc_event_record": (i32, [P, P]),
             "bpc_event_elapsed_ms": (i32, [P, P, ctypes.POINTER(ctypes.c_float)]),
             "bpc_mem_info": (i32, [ctypes.POINTER(sz), ctypes.POINTER(sz)]),
+            "bpc_server_default_config": (None, [ctypes.POINTER(ServerConfig)]),
+            "bpc_server_create": (i32, [ctypes.POINTER(P), ctypes.POINTER(ServerConfig)]),
+            "bpc_server_submit": (i32, [P, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),
+            "bpc_server_wait": (i32, [P, ctypes.c_uint64, i32, ctypes.POINTER(Result)]),
+            "bpc_server_release": (i32, [P, ctypes.c_uint64]),
+            "bpc_server_stats": (i32, [P, ctypes.POINTER(ServerStats)]),
+            "bpc_server_destroy": (i32, [P]),
             "bpc_batch_create": (i32, [ctypes.POINTER(P), i32, ctypes.c_longlong]),
             "bpc_batch_destroy": (i32, [P]),
             "bpc_batch_set_tuning": (i32, [P, i32, i32]),
@@ -431,3 +461,57 @@ def make_image(src, dst, lut, size_x, size_y, p, ncw=None, fill_unused=1):
     """Image descriptor from DeviceBuffers and a Lut."""
     return Image(src.ptr, dst.ptr, lut.ptr, lut.len, ncw.ptr if ncw is not None else None, size_x, size_y, p,
                  fill_unused)
+
+
+# ---- continuous-batching server ------------------------------------------------------
+
+class Server:
+    """Request server: submit encode/decode requests on device buffers, wait for results."""
+
+    def __init__(self, max_inflight=2, max_batch_images=64, max_batch_codeblocks=1 << 18, hbm_budget_bytes=32 << 30,
+                 waves_per_block=4):
+        cfg = ServerConfig()
+        lib().bpc_server_default_config(ctypes.byref(cfg))
+        cfg.max_inflight, cfg.max_batch_images = max_inflight, max_batch_images
+        cfg.max_batch_codeblocks, cfg.hbm_budget_bytes = max_batch_codeblocks, hbm_budget_bytes
+        cfg.waves_per_block = waves_per_block
+        p = ctypes.c_void_p()
+        _check(lib().bpc_server_create(ctypes.byref(p), ctypes.byref(cfg)))
+        self.ptr = p
+
+    def submit(self, direction, src, lut, size_x, size_y, p):
+        req = Request(direction, src.ptr if hasattr(src, "ptr") else src, lut.ptr, lut.len, size_x, size_y, p)
+        rid = ctypes.c_uint64()
+        _check(lib().bpc_server_submit(self.ptr, ctypes.byref(req), ctypes.byref(rid)))
+        return rid.value
+
+    def wait(self, rid, timeout_ms=-1):
+        res = Result()
+        _check(lib().bpc_server_wait(self.ptr, rid, timeout_ms, ctypes.byref(res)))
+        return res
+
+    def release(self, rid):
+        _check(lib().bpc_server_release(self.ptr, rid))
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().bpc_server_stats(self.ptr, ctypes.byref(st)))
+        return st
+
+    def destroy(self):
+        if self.ptr is not None:
+            lib().bpc_server_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def download_ptr(ptr, n, dtype=np.int32, stream=None):
+    """Copies n elements from a raw device pointer to a new host array."""
+    out = np.empty(n, dtype=dtype)
+    _check(lib().bpc_copy_d2h(_ptr(out), ctypes.c_void_p(ptr), out.nbytes, _sp(stream)))
+    return out

@@ -1,0 +1,370 @@
+// Continuous-batching request server over the batched codec.
+//
+// Requests (encode or decode of one image, device-resident inputs) are queued on
+// arrival.  A scheduler thread admits them in arrival order while the HBM budget
+// allows their output buffers, groups admitted requests of one direction into a
+// batch and launches it on one of several streams.  Up to `max_inflight` batches
+// run on the device at once, so requests that arrive while others are on the GPU
+// start as soon as a slot and memory free up, and their work overlaps on the
+// device.  Each request's output is the same as coding its image alone.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/bpc_paco.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Req {
+  uint64_t id = 0;
+  bpc_request r;
+  size_t out_bytes = 0;
+  void* out = nullptr;  // server-allocated output (encode: codestream, decode: coefficients)
+  int32_t* ncw = nullptr;
+  Clock::time_point t_submit, t_start, t_done;
+  int status = BPC_OK;
+  bool admitted = false;  // its output bytes count against the HBM budget
+  bool done = false;
+};
+
+struct Slot {
+  bpc_batch* batch = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev = nullptr;
+  uint32_t* d_status = nullptr;
+  std::vector<Req*> reqs;
+  bool busy = false;
+};
+
+int64_t ncb_of(const bpc_request& r) {
+  return (int64_t)(r.size_x / 64) * (r.size_y / std::max(1, r.params.cblock_length));
+}
+
+}  // namespace
+
+struct bpc_server {
+  bpc_server_config cfg;
+  int device = 0;
+  std::vector<Slot> slots;
+  std::mutex mu;
+  std::condition_variable cv_work;   // scheduler wake-up
+  std::condition_variable cv_done;   // completion notifications
+  std::deque<Req*> pending;
+  std::map<uint64_t, Req*> all;
+  uint64_t next_id = 1;
+  size_t committed = 0;  // bytes of outputs held by admitted, not yet released requests
+  // released output buffers kept for reuse, by size (hipFree would synchronize the device)
+  std::multimap<size_t, void*> pool;
+  size_t pooled = 0;
+  std::atomic<bool> stop{false};
+  std::thread th;
+  // statistics
+  uint64_t batches = 0, completed = 0;
+  double samples_done = 0;
+  size_t max_batch_images_seen = 0;
+};
+
+namespace {
+
+void finish_slot(bpc_server* s, Slot& sl) {
+  const int st = bpc_status_take(sl.d_status, sl.stream);  // the batch is done: no wait
+  const auto now = Clock::now();
+  std::lock_guard<std::mutex> lk(s->mu);
+  for (Req* q : sl.reqs) {
+    q->t_done = now;
+    q->status = st;
+    q->done = true;
+    s->completed++;
+    s->samples_done += (double)q->r.size_x * q->r.size_y;
+  }
+  sl.reqs.clear();
+  sl.busy = false;
+  s->cv_done.notify_all();
+}
+
+// Takes admissible requests of one direction off the queue (arrival order); the
+// first pending request fixes the direction and the codeblock length.
+std::vector<Req*> take_batch(bpc_server* s) {
+  std::vector<Req*> out;
+  if (s->pending.empty()) return out;
+  const int dir = s->pending.front()->r.direction;
+  const int rows = s->pending.front()->r.params.cblock_length;
+  int64_t cbs = 0;
+  for (auto it = s->pending.begin(); it != s->pending.end() && (int)out.size() < s->cfg.max_batch_images;) {
+    Req* q = *it;
+    const int64_t n = ncb_of(q->r);
+    const bool fits_mem = s->committed + q->out_bytes <= s->cfg.hbm_budget_bytes;
+    if (q->r.direction != dir || q->r.params.cblock_length != rows) {
+      ++it;
+      continue;
+    }
+    if (!fits_mem || cbs + n > s->cfg.max_batch_codeblocks) break;  // admission in arrival order
+    cbs += n;
+    s->committed += q->out_bytes;
+    q->admitted = true;
+    out.push_back(q);
+    it = s->pending.erase(it);
+  }
+  return out;
+}
+
+size_t ncw_bytes(const bpc_request& r) { return sizeof(int32_t) * (size_t)std::max<int64_t>(1, ncb_of(r)); }
+
+void* pool_get(bpc_server* s, size_t bytes) {
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->pool.find(bytes);
+    if (it != s->pool.end()) {
+      void* p = it->second;
+      s->pool.erase(it);
+      s->pooled -= bytes;
+      return p;
+    }
+  }
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+  return p;
+}
+
+// caller holds s->mu
+void pool_put(bpc_server* s, void* p, size_t bytes) {
+  if (!p) return;
+  if (s->pooled + bytes > s->cfg.hbm_budget_bytes) {
+    (void)hipFree(p);
+    return;
+  }
+  s->pool.emplace(bytes, p);
+  s->pooled += bytes;
+}
+
+int launch_batch(bpc_server* s, Slot& sl, std::vector<Req*>& reqs) {
+  std::vector<bpc_image> imgs(reqs.size());
+  const auto now = Clock::now();
+  for (size_t k = 0; k < reqs.size(); ++k) {
+    Req* q = reqs[k];
+    if (!q->out) q->out = pool_get(s, q->out_bytes);
+    if (!q->out) return BPC_ERR_HIP;
+    if (q->r.direction == BPC_CODE && !q->ncw) q->ncw = (int32_t*)pool_get(s, ncw_bytes(q->r));
+    if (q->r.direction == BPC_CODE && !q->ncw) return BPC_ERR_HIP;
+    q->t_start = now;
+    bpc_image& im = imgs[k];
+    im.src = q->r.src;
+    im.dst = (int32_t*)q->out;
+    im.lut = q->r.lut;
+    im.lut_len = q->r.lut_len;
+    im.ncw = q->r.direction == BPC_CODE ? q->ncw : nullptr;
+    im.size_x = q->r.size_x;
+    im.size_y = q->r.size_y;
+    im.params = q->r.params;
+    im.fill_unused = 1;
+  }
+  int st = bpc_batch_run(sl.batch, reqs.front()->r.direction, imgs.data(), (int)imgs.size(), 0, sl.d_status,
+                         sl.stream);
+  if (st) return st;
+  if (hipEventRecord(sl.ev, sl.stream) != hipSuccess) return BPC_ERR_HIP;
+  sl.reqs = reqs;
+  sl.busy = true;
+  return BPC_OK;
+}
+
+void scheduler(bpc_server* s) {
+  (void)hipSetDevice(s->device);
+  while (!s->stop.load()) {
+    bool progressed = false;
+    for (Slot& sl : s->slots) {  // completions
+      if (sl.busy && hipEventQuery(sl.ev) == hipSuccess) {
+        finish_slot(s, sl);
+        progressed = true;
+      }
+    }
+    for (Slot& sl : s->slots) {  // launches
+      if (sl.busy) continue;
+      std::vector<Req*> reqs;
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        reqs = take_batch(s);
+        if (!reqs.empty()) {
+          s->batches++;
+          s->max_batch_images_seen = std::max(s->max_batch_images_seen, reqs.size());
+        }
+      }
+      if (reqs.empty()) break;
+      int st = launch_batch(s, sl, reqs);
+      if (st) {  // fail these requests
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (Req* q : reqs) {
+          q->status = st;
+          q->done = true;
+          q->t_done = Clock::now();
+        }
+        s->cv_done.notify_all();
+      }
+      progressed = true;
+    }
+    if (!progressed) {
+      std::unique_lock<std::mutex> lk(s->mu);
+      bool any_busy = false;
+      for (Slot& sl : s->slots) any_busy |= sl.busy;
+      if (any_busy)
+        s->cv_work.wait_for(lk, std::chrono::microseconds(20));
+      else
+        s->cv_work.wait_for(lk, std::chrono::milliseconds(5), [&] { return s->stop.load() || !s->pending.empty(); });
+    }
+  }
+  for (Slot& sl : s->slots)
+    if (sl.busy) {
+      (void)hipEventSynchronize(sl.ev);
+      finish_slot(s, sl);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void bpc_server_default_config(bpc_server_config* c) {
+  if (!c) return;
+  c->max_inflight = 2;
+  c->max_batch_images = 64;
+  c->max_batch_codeblocks = 1 << 18;
+  c->hbm_budget_bytes = (size_t)32 << 30;
+  c->waves_per_block = 4;
+}
+
+int bpc_server_create(bpc_server** out, const bpc_server_config* cfg) {
+  if (!out) return BPC_ERR_ARG;
+  bpc_server_config c;
+  bpc_server_default_config(&c);
+  if (cfg) c = *cfg;
+  if (c.max_inflight < 1 || c.max_inflight > 8 || c.max_batch_images < 1 || c.max_batch_images > 4096 ||
+      c.max_batch_codeblocks < 1)
+    return BPC_ERR_ARG;
+  bpc_server* s = new bpc_server();
+  s->cfg = c;
+  if (hipGetDevice(&s->device) != hipSuccess) {
+    delete s;
+    return BPC_ERR_NODEVICE;
+  }
+  s->slots.resize(c.max_inflight);
+  for (Slot& sl : s->slots) {
+    int st = bpc_batch_create(&sl.batch, c.max_batch_images, c.max_batch_codeblocks);
+    if (!st) st = bpc_batch_set_tuning(sl.batch, c.waves_per_block, 0);
+    if (!st && hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess) st = BPC_ERR_HIP;
+    if (!st && hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess) st = BPC_ERR_HIP;
+    if (!st && hipMalloc(&sl.d_status, sizeof(uint32_t)) != hipSuccess) st = BPC_ERR_HIP;
+    if (!st && hipMemset(sl.d_status, 0, sizeof(uint32_t)) != hipSuccess) st = BPC_ERR_HIP;
+    if (st) {
+      s->stop = true;
+      bpc_server_destroy(s);
+      return st;
+    }
+  }
+  s->th = std::thread(scheduler, s);
+  *out = s;
+  return BPC_OK;
+}
+
+int bpc_server_submit(bpc_server* s, const bpc_request* r, uint64_t* id) {
+  if (!s || !r || !id) return BPC_ERR_ARG;
+  if (r->direction != BPC_CODE && r->direction != BPC_DECODE) return BPC_ERR_ARG;
+  if (r->size_x <= 0 || r->size_y <= 0 || r->size_x % 64 || r->params.cblock_length < 2 ||
+      r->size_y % r->params.cblock_length)
+    return BPC_ERR_ARG;
+  Req* q = new Req();
+  q->r = *r;
+  q->out_bytes = r->direction == BPC_CODE ? (size_t)bpc_stream_words(r->size_x, r->size_y, &r->params) * 4
+                                          : (size_t)r->size_x * r->size_y * 4;
+  if (q->out_bytes > s->cfg.hbm_budget_bytes) {
+    delete q;
+    return BPC_ERR_ARG;
+  }
+  q->t_submit = Clock::now();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    q->id = s->next_id++;
+    s->all[q->id] = q;
+    s->pending.push_back(q);
+  }
+  s->cv_work.notify_one();
+  *id = q->id;
+  return BPC_OK;
+}
+
+int bpc_server_wait(bpc_server* s, uint64_t id, int timeout_ms, bpc_result* res) {
+  if (!s || !res) return BPC_ERR_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->all.find(id);
+  if (it == s->all.end()) return BPC_ERR_ARG;
+  Req* q = it->second;
+  if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms < 0 ? 1 << 30 : timeout_ms),
+                           [&] { return q->done; }))
+    return BPC_ERR_TIMEOUT;
+  res->status = q->status;
+  res->output = q->out;
+  res->ncw = q->ncw;
+  res->queue_us = std::chrono::duration<double, std::micro>(q->t_start - q->t_submit).count();
+  res->latency_us = std::chrono::duration<double, std::micro>(q->t_done - q->t_submit).count();
+  return BPC_OK;
+}
+
+int bpc_server_release(bpc_server* s, uint64_t id) {
+  if (!s) return BPC_ERR_ARG;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(id);
+    if (it == s->all.end() || !it->second->done) return BPC_ERR_ARG;
+    Req* q = it->second;
+    s->all.erase(it);
+    if (q->admitted) s->committed -= q->out_bytes;
+    pool_put(s, q->out, q->out_bytes);
+    if (q->ncw) pool_put(s, q->ncw, ncw_bytes(q->r));
+    delete q;
+  }
+  s->cv_work.notify_one();
+  return BPC_OK;
+}
+
+int bpc_server_stats(bpc_server* s, bpc_server_stats_t* st) {
+  if (!s || !st) return BPC_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  st->batches = s->batches;
+  st->completed = s->completed;
+  st->pending = s->pending.size();
+  st->committed_bytes = s->committed;
+  st->max_batch_images = s->max_batch_images_seen;
+  return BPC_OK;
+}
+
+int bpc_server_destroy(bpc_server* s) {
+  if (!s) return BPC_OK;
+  s->stop = true;
+  s->cv_work.notify_all();
+  if (s->th.joinable()) s->th.join();
+  for (auto& kv : s->all) {
+    if (kv.second->out) (void)hipFree(kv.second->out);
+    if (kv.second->ncw) (void)hipFree(kv.second->ncw);
+    delete kv.second;
+  }
+  for (auto& kv : s->pool) (void)hipFree(kv.second);
+  for (Slot& sl : s->slots) {
+    if (sl.batch) bpc_batch_destroy(sl.batch);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    if (sl.ev) (void)hipEventDestroy(sl.ev);
+    if (sl.d_status) (void)hipFree(sl.d_status);
+  }
+  delete s;
+  return BPC_OK;
+}
+
+}  // extern "C"

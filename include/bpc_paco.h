@@ -29,6 +29,7 @@ extern "C" {
 #define BPC_ERR_WIDE (-6)      /* device-resident call: a codeblock's MSB is beyond the kernel's bitplane
                                   registers (only when MSB >= lut_n_bitplanes); bpc_run_ex retries itself */
 #define BPC_ERR_NODEVICE (-7)  /* no HIP device */
+#define BPC_ERR_TIMEOUT (-8)   /* bpc_server_wait: the request did not finish in time */
 
 /* ---- direction (enum BPC, common.h:8-11) ---- */
 #define BPC_CODE 0
@@ -141,6 +142,56 @@ int bpc_batch_set_tuning(bpc_batch* batch, int waves_per_block, int waves_per_si
  * afterwards.  Errors land in d_status (bpc_status_take). */
 int bpc_batch_run(bpc_batch* batch, int direction, const bpc_image* images, int n_images, int use_graph,
                   uint32_t* d_status, void* stream);
+
+/* ---- continuous-batching server ------------------------------------------------
+ * Encode / decode requests on device-resident inputs are queued on arrival.  A
+ * scheduler thread admits them in arrival order while the HBM budget allows their
+ * output buffers, groups admitted requests of one direction into a batch (the
+ * batched codec above) and launches it on one of max_inflight streams, so
+ * requests that arrive while others are on the GPU start as soon as a slot and
+ * memory free up and their work overlaps on the device.  Outputs are allocated by
+ * the server and stay in HBM until bpc_server_release.  No counterpart in the
+ * reference (one synchronous image per CUDA_BPC call). */
+typedef struct bpc_server_config {
+  int32_t max_inflight;         /* batches on the device at once (streams), 1..8 */
+  int32_t max_batch_images;     /* images per batch */
+  long long max_batch_codeblocks;
+  size_t hbm_budget_bytes;      /* output bytes held by admitted, unreleased requests */
+  int32_t waves_per_block;      /* tuning of the batched kernel (1, 2 or 4) */
+} bpc_server_config;
+
+typedef struct bpc_request {
+  int32_t direction;   /* BPC_CODE or BPC_DECODE */
+  const int32_t* src;  /* device: coefficients (encode) or codestream regions (decode) */
+  const int32_t* lut;
+  int32_t lut_len;
+  int32_t size_x, size_y;
+  bpc_params params;
+} bpc_request;
+
+typedef struct bpc_result {
+  int32_t status;      /* BPC_OK or an error of the request's batch */
+  void* output;        /* device: codestream regions (encode) or coefficients (decode) */
+  int32_t* ncw;        /* device: codeword slots per codeblock (encode) */
+  double queue_us;     /* submit -> batch launch */
+  double latency_us;   /* submit -> completion seen by the scheduler */
+} bpc_result;
+
+typedef struct bpc_server_stats_t {
+  unsigned long long batches, completed, pending;
+  size_t committed_bytes, max_batch_images;
+} bpc_server_stats_t;
+
+typedef struct bpc_server bpc_server;
+void bpc_server_default_config(bpc_server_config* cfg);
+int bpc_server_create(bpc_server** server, const bpc_server_config* cfg);
+int bpc_server_submit(bpc_server* server, const bpc_request* req, uint64_t* id);
+/* Blocks until request id finished (timeout_ms < 0: no limit); fills res. */
+int bpc_server_wait(bpc_server* server, uint64_t id, int timeout_ms, bpc_result* res);
+/* Returns the request's output buffers to the server (after bpc_server_wait). */
+int bpc_server_release(bpc_server* server, uint64_t id);
+int bpc_server_stats(bpc_server* server, bpc_server_stats_t* stats);
+int bpc_server_destroy(bpc_server* server);
 
 /* ---- device memory / stream / event helpers (thin HIP wrappers) ---- */
 int bpc_dev_malloc(void** ptr, size_t bytes);
