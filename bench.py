@@ -320,6 +320,85 @@ def bench_served(args):
     return out
 
 
+def bench_large(args, check=True):
+    """16384x16384 (65536 codeblocks, 1 GiB int32), 20 magnitude bitplanes (LL
+    uniform on [0, 2^19), detail scale 2^(14-d)), 3 passes, 8192-word regions,
+    batched persistent path; oracle on a seeded 1% codeblock sample."""
+    size, nbp, stride = 16384, 20, 8192
+    log("large_16384: generating image and LUT")
+    img = synth.synth_image(size, size, 5, args.seed + 9000, ll_bits=19, detail_exp=14, mag_bits=nbp)
+    train = synth.synth_image(size // 4, size // 4, 5, args.seed + 9001, ll_bits=17, detail_exp=12, mag_bits=nbp)
+    lut_arr = bpc.train_lut(train, bpc.params(cpass3=1, lut_n_bitplanes=nbp))
+    del train
+    p = bpc.params(cpass3=1, lut_n_bitplanes=nbp, cb_stride_words=stride, layout=bpc.LAYOUT_TILED)
+    ncb = p.n_codeblocks(size, size)
+    lut = bpc.Lut.from_array(lut_arr)
+    d_in = bpc.DeviceBuffer.from_array(synth.to_tiled(img))
+    d_cs = bpc.DeviceBuffer(bpc.stream_words(size, size, p))
+    d_out = bpc.DeviceBuffer(size * size)
+    d_ncw = bpc.DeviceBuffer(ncb)
+    d_status = bpc.DeviceBuffer(1, np.uint32)
+    d_status.fill_bytes(0)
+    stream = bpc.Stream()
+    bpc.device_sync()
+    batch = bpc.Batch(1, ncb)
+    enc = [bpc.make_image(d_in, d_cs, lut, size, size, p, d_ncw)]
+    dec = [bpc.make_image(d_cs, d_out, lut, size, size, p)]
+    ev = [bpc.Event() for _ in range(3)]
+
+    def step():
+        ev[0].record(stream)
+        batch.run(bpc.CODE, enc, use_graph=True, d_status=d_status, stream=stream)
+        ev[1].record(stream)
+        batch.run(bpc.DECODE, dec, use_graph=True, d_status=d_status, stream=stream)
+        ev[2].record(stream)
+        return ev[0].elapsed_ms(ev[1]), ev[1].elapsed_ms(ev[2])
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    if bpc.status_take(d_status, stream) != 0:
+        raise RuntimeError("large codec status: %s" % bpc.lib().bpc_last_error().decode())
+    exact = None
+    if check:
+        rec = synth.from_tiled(d_out.download(stream=stream), size, size)
+        if not np.array_equal(rec, img):
+            raise RuntimeError("large round trip failed")
+        del rec
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        cs = d_cs.download(stream=stream).reshape(ncb, stride)
+        sample = np.sort(np.random.default_rng(args.seed).choice(ncb, size=ncb // 100, replace=False)).astype(np.int32)
+        st, ostream, _ = O.encode_list(img, sample, lut_arr, O.params(cpass3=1, lut_n_bitplanes=nbp,
+                                                                      cb_stride_words=stride))
+        ost = ostream.reshape(ncb, stride)
+        exact = bool(st == 0 and all(np.array_equal(cs[c], ost[c]) for c in sample))
+        del cs, ostream, ost
+        if not exact:
+            raise RuntimeError("large sample differs from the oracle")
+    steps = max(2, args.steps // 4)
+    times = []
+    bpc.device_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        e, d = step()
+        times.append((e, d))
+    bpc.device_sync()
+    wall = time.perf_counter() - t0
+    ncw = d_ncw.download(stream=stream)
+    out = {"workload": "large_16384 (65536 codeblocks, 20 bitplanes, 3-pass, 8192-word regions, batched path)",
+           "msamples_s": round(size * size * steps / wall / 1e6, 1),
+           "encode_ms": round(float(np.mean([e for e, _ in times])), 3),
+           "decode_ms": round(float(np.mean([d for _, d in times])), 3),
+           "bits_per_sample": round(float(ncw.sum()) * 16 / (size * size), 3),
+           "max_codewords_per_codeblock": int(ncw.max()),
+           "oracle_sample_codeblocks": ncb // 100 if check else 0, "bit_exact_sample_and_round_trip": exact}
+    batch.destroy()
+    for b in (d_in, d_cs, d_out, d_ncw, d_status):
+        b.free()
+    lut.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -327,8 +406,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle baseline and parity check")
-    ap.add_argument("--workloads", default="plain,batched,served",
-                    help="comma list of plain, batched, served (the plain workload always runs: it gives `value`)")
+    ap.add_argument("--workloads", default="plain,batched,served,large",
+                    help="comma list of plain, batched, served, large (plain always runs: it gives `value`)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB 5 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -462,6 +541,8 @@ def main():
         out["batched"] = bench_batched(args, check=not args.no_cpu)
     if "served" in wl:
         out["served"] = bench_served(args)
+    if "large" in wl:
+        out["large"] = bench_large(args, check=not args.no_cpu)
     print(json.dumps(out), flush=True)
 
 
