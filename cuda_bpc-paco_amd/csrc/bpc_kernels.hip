@@ -71,6 +71,14 @@ struct LaneState {
 // Lane mask of a predicate, taken straight from the compare (no 0/1 round trip).
 __device__ __forceinline__ u64 ballot(bool b) { return __builtin_amdgcn_ballot_w64(b); }
 
+// Hides a value from the optimizer so that a later compare on it stays a single
+// v_cmp whose lane mask feeds a ballot directly (instead of being folded into a
+// combination of masks that must be rebuilt through a 0/1 VGPR).
+__device__ __forceinline__ u32 opaque(u32 v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 __device__ __forceinline__ u32 dpp_from_left(u32 v) {  // lane l receives lane l-1
   return (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xF, 0xF, true);  // wave_shr:1
 }
@@ -104,7 +112,7 @@ struct Geo {
 template <int DEC, int MAXP>
 __device__ __forceinline__ void reserve(LaneState<MAXP>& s, bool at, const Geo& g, const int32_t* src) {
   // 0 exactly for the lanes that reserve: one compare feeds the ballot directly
-  const u32 szr = at ? s.Sz : 1u;
+  const u32 szr = opaque(at ? s.Sz : 1u);
   const u64 m = ballot(szr == 0);
   if (m) {
     const u32 lo = (u32)m, hi = (u32)(m >> 32);
@@ -129,13 +137,11 @@ __device__ __forceinline__ void reserve(LaneState<MAXP>& s, bool at, const Geo& 
 // Encoder coder step (BPC_PaCo.cu:281-294).
 template <int MAXP>
 __device__ __forceinline__ void enc_symbol(LaneState<MAXP>& s, u32 sym, u32 prob, const Geo& g, int32_t* dst) {
-  const u32 aux = (__umul24(s.Sz, prob) >> 7) + sym;  // Sz < 2^16, prob < 2^7
-  if (sym) {
-    s.Sz -= aux;
-    s.L += aux;
-  } else {
-    s.Sz = aux;
-  }
+  const u32 t = __umul24(s.Sz, prob) >> 7;  // Sz < 2^16, prob < 2^7
+  const u32 aux = t + sym;
+  const bool one = sym != 0;
+  s.Sz = one ? s.Sz - aux : t;
+  s.L = one ? s.L + aux : s.L;
   if (s.Sz == 0) dst[g.region + s.slot] = (int32_t)s.L;
 }
 
@@ -197,11 +203,13 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
     } else {
       sym = dec_symbol(s, prob);
     }
-    const u64 nb64 = (u64)sym << i;
-    s.S[c] |= nb64;
-    if (CLEAN) s.R[c] |= nb64;  // BPC_PaCo.cu:534
-    if (DEC) s.cur[c] |= nb64;  // magnitude bit i of this plane
+    if (sym) {
+      s.S[c] |= bit;
+      if (CLEAN) s.R[c] |= bit;  // BPC_PaCo.cu:534
+      if (DEC) s.cur[c] |= bit;  // magnitude bit i of this plane
+    }
   }
+  sym = opaque(sym);
   // site 2: sign symbol of the coefficients that became significant
   if (ballot(sym != 0)) {
     const bool nsig = sym != 0;
