@@ -399,6 +399,45 @@ def bench_large(args, check=True):
     return out
 
 
+def bench_train(args):
+    """LUT training on the device (symbol statistics of the encoder's modelling, no
+    coding) for the plain workload's training image: 4096x4096, 3 passes, 16 LUT
+    bitplanes; the host trainer on all host threads beside it, same table required."""
+    size, cp3, nbp = 4096, 1, 16
+    log("train_4096_3pass: generating the training image")
+    train = synth.synth_image(size, size, 5, args.seed + 1, ll_bits=13, detail_exp=9, mag_bits=16)
+    p = bpc.params(cpass3=cp3, lut_n_bitplanes=nbp)
+    n = bpc.lut_len(p)
+    stream = bpc.Stream()
+    d_img = bpc.DeviceBuffer.from_array(train)
+    d_lut = bpc.DeviceBuffer(n)
+    bpc.device_sync()
+    ev = [bpc.Event() for _ in range(2)]
+    times = []
+    for k in range(max(1, args.warmup) + max(3, args.steps // 2)):
+        ev[0].record(stream)
+        bpc.train_lut_device(d_img, size, size, p, d_lut=d_lut, stream=stream)
+        ev[1].record(stream)
+        ms = ev[0].elapsed_ms(ev[1])
+        if k >= max(1, args.warmup):
+            times.append(ms)
+    dev_lut = d_lut.download(stream=stream)
+    threads = os.cpu_count() or 1
+    t0 = time.perf_counter()
+    host_lut = bpc.train_lut(train, p, nthreads=threads)
+    host_s = time.perf_counter() - t0
+    exact = bool(np.array_equal(dev_lut, host_lut))
+    if not exact:
+        raise RuntimeError("device LUT differs from the host trainer's")
+    gpu_ms = float(np.median(times))
+    d_img.free()
+    d_lut.free()
+    return {"workload": "train_4096_3pass (4096x4096 training image, 16 LUT bitplanes, %d entries)" % n,
+            "gpu_ms": round(gpu_ms, 4), "gpu_msamples_s": round(size * size / gpu_ms / 1e3, 1),
+            "host_trainer_ms": round(host_s * 1e3, 2), "host_threads": threads,
+            "host_msamples_s": round(size * size / host_s / 1e6, 1), "identical_to_host_trainer": exact}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -406,8 +445,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle baseline and parity check")
-    ap.add_argument("--workloads", default="plain,batched,served,large",
-                    help="comma list of plain, batched, served, large (plain always runs: it gives `value`)")
+    ap.add_argument("--workloads", default="plain,batched,served,large,train",
+                    help="comma list of plain, batched, served, large, train (plain always runs: it gives `value`)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB 5 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -543,6 +582,8 @@ def main():
         out["served"] = bench_served(args)
     if "large" in wl:
         out["large"] = bench_large(args, check=not args.no_cpu)
+    if "train" in wl:
+        out["lut_trainer"] = bench_train(args)
     print(json.dumps(out), flush=True)
 
 

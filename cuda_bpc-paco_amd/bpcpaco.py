@@ -131,6 +131,7 @@ def lib():
             "bpc_lut_upload": (i32, [P, i32, ctypes.POINTER(P)]),
             "bpc_lut_free": (i32, [P]),
             "bpc_lut_train": (i32, [P, i32, i32, PP, P, i32, i32]),
+            "bpc_lut_train_device": (i32, [P, i32, i32, PP, P, P, P, P]),
             "bpc_encode_device": (i32, [P, P, i32, i32, P, i32, PP, i32, P, P, P]),
             "bpc_decode_device": (i32, [P, P, i32, i32, P, i32, PP, P, P]),
             "bpc_status_take": (i32, [P, P]),
@@ -320,6 +321,19 @@ class Lut:
         _check(lib().bpc_lut_upload(_ptr(arr), arr.size, ctypes.byref(p)))
         return cls(p, arr.size, host=arr)
 
+    @classmethod
+    def train_device(cls, d_img, size_x, size_y, p, stream=None):
+        """LUT trained on the device from a device-resident training image (p.layout)."""
+        ptr = ctypes.c_void_p()
+        n = lut_len(p)
+        _check(lib().bpc_dev_malloc(ctypes.byref(ptr), n * 4))
+        try:
+            train_lut_device(d_img, size_x, size_y, p, d_lut=ptr, stream=stream)
+        except BPCError:
+            lib().bpc_lut_free(ptr)
+            raise
+        return cls(ptr, n, p.lut_n_bitplanes)
+
     def free(self):
         if self.ptr is not None:
             lib().bpc_lut_free(self.ptr)
@@ -344,6 +358,40 @@ def train_lut(img, p, nthreads=0):
     out = np.zeros(n, dtype=np.int32)
     _check(lib().bpc_lut_train(_ptr(img), X, Y, ctypes.byref(p), _ptr(out), n, nthreads))
     return out
+
+
+def _dptr(b):
+    if b is None:
+        return None
+    return b.ptr if hasattr(b, "ptr") else b
+
+
+def train_lut_device(d_img, size_x, size_y, p, d_lut=None, d_n0=None, d_n=None, stream=None):
+    """Device trainer (asynchronous): LUT and/or per-entry counts from a device image."""
+    _check(lib().bpc_lut_train_device(_dptr(d_img), size_x, size_y, ctypes.byref(p), _dptr(d_lut), _dptr(d_n0),
+                                      _dptr(d_n), _sp(stream)))
+
+
+def train_lut_gpu(img, p, counts=False):
+    """Device trainer on a host image (2-D raster, or flattened codeblock-contiguous data
+    with p.layout tiled and shape (size_y, size_x)).  Returns the LUT, or (lut, n0, n)."""
+    img = np.ascontiguousarray(img, dtype=np.int32)
+    Y, X = img.shape
+    n = lut_len(p)
+    d_img = DeviceBuffer.from_array(img)
+    d_lut = DeviceBuffer(n)
+    d_n0 = DeviceBuffer(n, np.uint32) if counts else None
+    d_n = DeviceBuffer(n, np.uint32) if counts else None
+    try:
+        train_lut_device(d_img, X, Y, p, d_lut, d_n0, d_n)
+        lut = d_lut.download()
+        if counts:
+            return lut, d_n0.download(), d_n.download()
+        return lut
+    finally:
+        for b in (d_img, d_lut, d_n0, d_n):
+            if b is not None:
+                b.free()
 
 
 # ---- codec -------------------------------------------------------------------------

@@ -119,6 +119,33 @@ struct DevScratch {
 std::mutex g_scratch_mu;
 std::vector<DevScratch> g_scratch;
 
+// At least `need` bytes of the device's scratch for work about to be queued on
+// `stream`; the caller holds g_scratch_mu and ends with scratch_release.
+int scratch_acquire(int dev, size_t need, hipStream_t stream, void** ptr) {
+  if ((int)g_scratch.size() <= dev) g_scratch.resize(dev + 1);
+  DevScratch& sc = g_scratch[dev];
+  if (!sc.last) HIP_TRY(hipEventCreateWithFlags(&sc.last, hipEventDisableTiming));
+  if (need > sc.bytes) {
+    if (sc.has_last) HIP_TRY(hipEventSynchronize(sc.last));
+    if (sc.ptr) HIP_TRY(hipFree(sc.ptr));
+    sc.ptr = nullptr;
+    sc.bytes = 0;
+    HIP_TRY(hipMalloc(&sc.ptr, need));
+    sc.bytes = need;
+  } else if (sc.has_last) {
+    HIP_TRY(hipStreamWaitEvent(stream, sc.last, 0));
+  }
+  *ptr = sc.ptr;
+  return BPC_OK;
+}
+
+int scratch_release(int dev, hipStream_t stream) {
+  DevScratch& sc = g_scratch[dev];
+  HIP_TRY(hipEventRecord(sc.last, stream));
+  sc.has_last = true;
+  return BPC_OK;
+}
+
 int launch(int decode, const int32_t* src, int32_t* dst, int size_x, int size_y, const int32_t* d_lut, int lut_len,
            const bpc_params* p, int fill, int32_t* d_ncw, uint32_t* d_status, hipStream_t stream, int maxp) {
   bpc::KArgs a;
@@ -147,25 +174,11 @@ int launch(int decode, const int32_t* src, int32_t* dst, int size_x, int size_y,
   const int grid = (int)std::min<long long>(pairs, (long long)per_cu * device_cus(dev));
   const size_t need = (size_t)grid * bpc::scratch_bytes_per_wave(maxp);
   std::lock_guard<std::mutex> lk(g_scratch_mu);
-  if ((int)g_scratch.size() <= dev) g_scratch.resize(dev + 1);
-  DevScratch& sc = g_scratch[dev];
-  if (!sc.last) HIP_TRY(hipEventCreateWithFlags(&sc.last, hipEventDisableTiming));
-  if (need > sc.bytes) {
-    if (sc.has_last) HIP_TRY(hipEventSynchronize(sc.last));
-    if (sc.ptr) HIP_TRY(hipFree(sc.ptr));
-    sc.ptr = nullptr;
-    sc.bytes = 0;
-    HIP_TRY(hipMalloc(&sc.ptr, need));
-    sc.bytes = need;
-  } else if (sc.has_last) {
-    HIP_TRY(hipStreamWaitEvent(stream, sc.last, 0));
-  }
-  a.scratch = sc.ptr;
+  int st = scratch_acquire(dev, need, stream, &a.scratch);
+  if (st) return st;
   hipError_t e = bpc::launch_codec(decode, p->cpass3, maxp, a, grid, stream);
   if (e != hipSuccess) return hip_fail(e, "kernel launch");
-  HIP_TRY(hipEventRecord(sc.last, stream));
-  sc.has_last = true;
-  return BPC_OK;
+  return scratch_release(dev, stream);
 }
 
 int check_device_call(const void* a, const void* b, int size_x, int size_y, const int32_t* d_lut, int lut_len,
@@ -544,6 +557,53 @@ int bpc_lut_load_text(const char* folder, int cpass3, int dwt_levels, int32_t** 
   if (lut_len) *lut_len = len;
   if (nbp) *nbp = n;
   return BPC_OK;
+}
+
+int bpc_lut_train_device(const int32_t* d_img, int size_x, int size_y, const bpc_params* p, int32_t* d_lut,
+                         uint32_t* d_n0, uint32_t* d_n, void* stream) {
+  int st = check_params(p);
+  if (st) return st;
+  st = check_image(size_x, size_y, p);
+  if (st) return st;
+  if (!d_img) return fail(BPC_ERR_ARG, "null image");
+  if (!d_lut && !d_n0 && !d_n) return fail(BPC_ERR_ARG, "no output buffer");
+  if (bpc_device_count() <= 0) return fail(BPC_ERR_NODEVICE, "no HIP device");
+  const int len = bpc_lut_len(p);
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  int max_lds = 0;
+  HIP_TRY(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  const size_t lds = (size_t)len * sizeof(unsigned long long);
+  if (lds > (size_t)max_lds)
+    return fail(BPC_ERR_ARG, "LUT of %d entries exceeds the device trainer's LDS histogram (%d entries); use bpc_lut_train",
+                len, max_lds / (int)sizeof(unsigned long long));
+  bpc::TrainArgs a;
+  a.img = d_img;
+  a.size_x = size_x;
+  a.size_y = size_y;
+  a.nx = size_x / 64;
+  a.ncb = (size_x / 64) * (size_y / p->cblock_length);
+  a.rows = p->cblock_length;
+  a.nbp = p->lut_n_bitplanes;
+  a.levels = p->dwt_levels;
+  a.lut_len = len;
+  a.layout = p->layout;
+  int per_cu = bpc::train_resident_blocks_per_cu(p->cpass3, (int)lds);
+  if (per_cu <= 0) per_cu = 1;
+  const long long pairs = (a.ncb + 1) / 2;
+  const int grid = (int)std::max<long long>(
+      1, std::min<long long>((pairs + bpc::TRAIN_WAVES - 1) / bpc::TRAIN_WAVES, (long long)per_cu * device_cus(dev)));
+  const size_t plane_bytes = (size_t)grid * bpc::TRAIN_WAVES * bpc::scratch_bytes_per_wave(bpc::TRAIN_MAXP);
+  const size_t need = plane_bytes + (size_t)grid * len * sizeof(unsigned long long);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  void* base = nullptr;
+  st = scratch_acquire(dev, need, (hipStream_t)stream, &base);
+  if (st) return st;
+  a.scratch = base;
+  a.partial = reinterpret_cast<unsigned long long*>(static_cast<char*>(base) + plane_bytes);
+  hipError_t e = bpc::launch_train(p->cpass3, a, grid, d_lut, d_n0, d_n, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "trainer launch");
+  return scratch_release(dev, (hipStream_t)stream);
 }
 
 int bpc_lut_free(int32_t* d_lut) {

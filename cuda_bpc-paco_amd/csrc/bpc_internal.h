@@ -76,10 +76,20 @@ struct BatchArgs {
   int rows;
 };
 
+// Symbol statistics of a training image (LUT trainer).
+struct TrainArgs {
+  const int32_t* img;  // coefficients (raster or codeblock-contiguous, as layout says)
+  int size_x, size_y, nx, ncb, rows, nbp, levels, lut_len, layout;
+  void* scratch;                  // per wave of the grid: scratch_bytes_per_wave(TRAIN_MAXP)
+  unsigned long long* partial;    // per workgroup: lut_len packed counts (symbols << 32 | zeros)
+};
+constexpr int TRAIN_MAXP = 28;   // every bitplane a coefficient below 2^28 can have
+constexpr int TRAIN_WAVES = 8;   // waves per trainer workgroup (one shared LDS histogram)
+
 // Kernel variants hold this many bitplanes in registers.
 int planes_for(int nbp);
 // Bitplane scratch of one wave: 2 row halves x MAXP planes x 64 lanes x 8 bytes.
-inline size_t scratch_bytes_per_wave(int maxp) { return (size_t)2 * maxp * 64 * 8; }
+constexpr size_t scratch_bytes_per_wave(int maxp) { return (size_t)2 * maxp * 64 * 8; }
 // Waves of the per-image kernel that fit on one CU (occupancy query).
 int codec_resident_waves_per_cu(int decode, int cpass3, int maxp);
 hipError_t launch_codec(int decode, int cpass3, int maxp, const KArgs& a, int grid_waves, hipStream_t stream);
@@ -89,6 +99,12 @@ hipError_t launch_batch(int decode, int maxp, const BatchArgs& a, int grid_block
 hipError_t launch_status_fold(uint32_t* ctl, uint32_t* status, hipStream_t stream);
 // Workgroups of the batched kernel that fit on one CU (occupancy query).
 int batch_resident_blocks_per_cu(int decode, int maxp, int waves_per_block);
+// Trainer workgroups that fit on one CU with lds_bytes of LDS each.
+int train_resident_blocks_per_cu(int cpass3, int lds_bytes);
+// Trainer over grid_blocks workgroups, then the LUT (and, if non-null, the summed
+// counts) from the partial rows.
+hipError_t launch_train(int cpass3, const TrainArgs& a, int grid_blocks, int32_t* lut, uint32_t* n0, uint32_t* n,
+                        hipStream_t stream);
 // 16-byte aligned device-to-device copy of a multiple of 16 bytes.
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
 

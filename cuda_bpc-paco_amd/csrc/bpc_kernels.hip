@@ -58,6 +58,10 @@ struct PlaneLut {  // probabilities of one bitplane, 7 bits each
   u32 sign;        // 4 sign contexts, entry k at bits 8k
   u32 cpsign;
   u32 ref;         // refinement probability
+  // symbol-count mode: LUT entries of context 0 of this plane, cleanup-pass offset,
+  // and whether the plane has LUT rows (planes >= the LUT's bitplanes are not counted)
+  int isig, isign, iref, icp;
+  bool tal;
 };
 
 template <int MAXP>
@@ -103,7 +107,13 @@ struct Geo {
   int rows;
   u32 edge_even;   // 7, or 0 for the first lane of a half (no left neighbour in the codeblock)
   u32 edge_odd;    // 7, or 0 for the last lane of a half (no right neighbour)
+  unsigned long long* hist;  // symbol-count mode: per LUT entry, (symbols << 32) + zeros, in LDS
 };
+
+// Symbol-count mode sink: one symbol coded with the probability of LUT entry idx.
+__device__ __forceinline__ void tally(unsigned long long* hist, int idx, u32 sym) {
+  atomicAdd(hist + idx, (1ull << 32) | (sym ? 0ull : 1ull));
+}
 
 // Slot reservation for one coder call site (BPC_PaCo.cu:268-277 / :304-316):
 // lanes at the site whose interval is exhausted take consecutive slots in lane
@@ -162,7 +172,7 @@ __device__ __forceinline__ u32 dec_symbol(LaneState<MAXP>& s, u32 prob) {
 // One scan step of the significance pass (CLEAN = 0, SPP_Encoder/SPP_Decoder,
 // BPC_PaCo.cu:412-507) or of the cleanup pass (CLEAN = 1, CP_Encoder/CP_Decoder,
 // BPC_PaCo.cu:510-589) at row i of column ODD.
-template <int ODD, bool FIRST, int DEC, int CP3, int CLEAN, int MAXP>
+template <int ODD, bool FIRST, int DEC, int CP3, int CLEAN, int TRAIN, int MAXP>
 __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u64 plane, const PlaneLut& pl,
                                          const Geo& g, const int32_t* src, int32_t* dst) {
   constexpr int c = ODD, o = 1 - ODD;
@@ -193,11 +203,14 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
     if (at) s.C[c] &= ~bit;            // BPC_PaCo.cu:528
   }
   // site 1: significance symbol
-  reserve<DEC>(s, at, g, src);
+  if (!TRAIN) reserve<DEC>(s, at, g, src);
   u32 sym = 0;
   if (at) {
     const u32 prob = (u32)((CLEAN ? pl.cpsig : pl.sig) >> (7u * ctx)) & 127u;
-    if (!DEC) {
+    if (TRAIN) {
+      sym = (u32)(plane >> i) & 1u;
+      if (pl.tal) tally(g.hist, (CLEAN ? pl.isig + pl.icp : pl.isig) + (int)ctx, sym);
+    } else if (!DEC) {
       sym = (u32)(plane >> i) & 1u;
       enc_symbol(s, sym, prob, g, dst);
     } else {
@@ -226,10 +239,14 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
     const u32 vs = (PAIR_SIGN_TABLE >> (2 * vidx)) & 3u;
     const u32 hs = (PAIR_SIGN_TABLE >> (2 * hidx)) & 3u;
     const u32 sctx = (u32)(SIGN_CTX_TABLE >> (4 * (3 * hs + vs))) & 0xFu;
-    reserve<DEC>(s, nsig, g, src);
+    if (!TRAIN) reserve<DEC>(s, nsig, g, src);
     if (nsig) {
       const u32 prob = ((CLEAN ? pl.cpsign : pl.sign) >> (8u * (sctx >> 1))) & 127u;
-      if (!DEC) {
+      if (TRAIN) {
+        if (pl.tal)
+          tally(g.hist, (CLEAN ? pl.isign + pl.icp : pl.isign) + (int)(sctx >> 1),
+                ((u32)(s.G[c] >> i) & 1u) ^ (sctx & 1u));
+      } else if (!DEC) {
         enc_symbol(s, ((u32)(s.G[c] >> i) & 1u) ^ (sctx & 1u), prob, g, dst);
       } else {
         const u32 v = dec_symbol(s, prob) ^ (sctx & 1u);
@@ -240,7 +257,7 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
 }
 
 // One scan step of the refinement pass (MRP_Encoder/MRP_Decoder, BPC_PaCo.cu:593-629).
-template <int ODD, int DEC, int MAXP>
+template <int ODD, int DEC, int TRAIN, int MAXP>
 __device__ __forceinline__ void ref_step(LaneState<MAXP>& s, int i, bool act, u64 plane, const PlaneLut& pl,
                                          const Geo& g, const int32_t* src, int32_t* dst) {
   constexpr int c = ODD;
@@ -248,9 +265,11 @@ __device__ __forceinline__ void ref_step(LaneState<MAXP>& s, int i, bool act, u6
   const bool r = (s.R[c] >> i) & 1u;
   const bool at = act && r;
   if (act && !r && ((s.S[c] >> i) & 1u)) s.R[c] |= bit;
-  reserve<DEC>(s, at, g, src);
+  if (!TRAIN) reserve<DEC>(s, at, g, src);
   if (at) {
-    if (!DEC) {
+    if (TRAIN) {
+      if (pl.tal) tally(g.hist, pl.iref, (u32)(plane >> i) & 1u);
+    } else if (!DEC) {
       enc_symbol(s, (u32)(plane >> i) & 1u, pl.ref, g, dst);
     } else {
       if (dec_symbol(s, pl.ref)) s.cur[c] |= bit;
@@ -274,7 +293,7 @@ __device__ __forceinline__ u64 wave_or(u64 x) {
 // and are supersets of the rows that take part: SPP codes only coefficients that
 // are not significant (significance only grows, one coefficient per step); CP
 // only those with the cleanup flag (never set during CP); MRP only significant ones.
-template <int DEC, int CP3, int CLEAN, int MAXP>
+template <int DEC, int CP3, int CLEAN, int TRAIN, int MAXP>
 __device__ __forceinline__ void sig_pass(LaneState<MAXP>& s, bool act, const u64* plane, const PlaneLut& pl,
                                          const Geo& g, const int32_t* src, int32_t* dst) {
   const u64 rowmask = g.rows >= 64 ? ~0ull : ((1ull << g.rows) - 1ull);
@@ -282,19 +301,19 @@ __device__ __forceinline__ void sig_pass(LaneState<MAXP>& s, bool act, const u64
   const u64 mO = wave_or(act ? (CLEAN ? s.C[1] : (~s.S[1] & rowmask)) : 0ull);
   u64 rows = mE | mO;
   if (rows & 1ull) {
-    if (mE & 1ull) sig_step<0, true, DEC, CP3, CLEAN>(s, 0, act, plane[0], pl, g, src, dst);
-    if (mO & 1ull) sig_step<1, true, DEC, CP3, CLEAN>(s, 0, act, plane[1], pl, g, src, dst);
+    if (mE & 1ull) sig_step<0, true, DEC, CP3, CLEAN, TRAIN>(s, 0, act, plane[0], pl, g, src, dst);
+    if (mO & 1ull) sig_step<1, true, DEC, CP3, CLEAN, TRAIN>(s, 0, act, plane[1], pl, g, src, dst);
   }
   rows &= ~1ull;
   while (rows) {
     const int i = __builtin_ctzll(rows);
     rows &= rows - 1ull;
-    if ((mE >> i) & 1ull) sig_step<0, false, DEC, CP3, CLEAN>(s, i, act, plane[0], pl, g, src, dst);
-    if ((mO >> i) & 1ull) sig_step<1, false, DEC, CP3, CLEAN>(s, i, act, plane[1], pl, g, src, dst);
+    if ((mE >> i) & 1ull) sig_step<0, false, DEC, CP3, CLEAN, TRAIN>(s, i, act, plane[0], pl, g, src, dst);
+    if ((mO >> i) & 1ull) sig_step<1, false, DEC, CP3, CLEAN, TRAIN>(s, i, act, plane[1], pl, g, src, dst);
   }
 }
 
-template <int DEC, int MAXP>
+template <int DEC, int TRAIN, int MAXP>
 __device__ __forceinline__ void ref_pass(LaneState<MAXP>& s, bool act, const u64* plane, const PlaneLut& pl,
                                          const Geo& g, const int32_t* src, int32_t* dst) {
   const u64 mE = wave_or(act ? s.S[0] : 0ull);
@@ -303,8 +322,8 @@ __device__ __forceinline__ void ref_pass(LaneState<MAXP>& s, bool act, const u64
   while (rows) {
     const int i = __builtin_ctzll(rows);
     rows &= rows - 1ull;
-    if ((mE >> i) & 1ull) ref_step<0, DEC>(s, i, act, plane[0], pl, g, src, dst);
-    if ((mO >> i) & 1ull) ref_step<1, DEC>(s, i, act, plane[1], pl, g, src, dst);
+    if ((mE >> i) & 1ull) ref_step<0, DEC, TRAIN>(s, i, act, plane[0], pl, g, src, dst);
+    if ((mO >> i) & 1ull) ref_step<1, DEC, TRAIN>(s, i, act, plane[1], pl, g, src, dst);
   }
 }
 
@@ -379,8 +398,12 @@ struct Scratch {
   }
 };
 
-template <int DEC, int CP3, int MAXP>
-__device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uint32_t* status_word, void* scr_base) {
+// TRAIN = 1 (with DEC = 0): symbol-count mode.  The encoder's modelling runs
+// unchanged, but every symbol is counted at the LUT entry whose probability would
+// code it (hist, in LDS) instead of being coded; nothing is written to dst.
+template <int DEC, int CP3, int MAXP, int TRAIN = 0>
+__device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uint32_t* status_word, void* scr_base,
+                                          unsigned long long* hist = nullptr) {
   const int lane = threadIdx.x & 63;
   const Scratch<MAXP> scr(scr_base, lane);
   const int half = lane >> 5;
@@ -399,6 +422,7 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
   g.rows = rows;
   g.edge_even = hl == 0 ? 0u : 7u;
   g.edge_odd = hl == 31 ? 0u : 7u;
+  g.hist = hist;
   int64_t cbase;
   int rs;
   if (a.layout) {
@@ -462,9 +486,9 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
 #pragma unroll
     for (int off = 1; off < 32; off <<= 1) mor |= (u32)__shfl_xor((int)mor, off, 64);
     msb = mor ? 31 - __clz(mor) : 32;
-    if (valid && hl == 0) a.dst[g.region] = msb;  // BPC_PaCo.cu:1155
+    if (!TRAIN && valid && hl == 0) a.dst[g.region] = msb;  // BPC_PaCo.cu:1155
     if (msb != 32 && msb > 27) {
-      status |= ST_RANGE;
+      if (!TRAIN) status |= ST_RANGE;  // symbol counting skips such a codeblock
       good = false;
     }
   } else {
@@ -487,7 +511,7 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
   const int sig0 = G * a.nbp + grp * a.nbp * N_SIG;
   const int sign0 = G * a.nbp * (1 + N_SIG) + grp * a.nbp * N_SIGN;
   const int cpoff = G * a.nbp * (N_SIG + N_SIGN);  // BPC_PaCo.cu:988-989
-  if (good) {
+  if (good && !TRAIN) {
     const int cp = CP3 ? cpoff : 0;
     if (ref0 + msb + 1 > a.lut_len || sig0 + cp + msb * N_SIG + N_SIG > a.lut_len ||
         sign0 + cp + msb * N_SIGN + N_SIGN > a.lut_len) {
@@ -518,16 +542,26 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
     PlaneLut pl;
     pl.sig = pl.cpsig = 0;
     pl.sign = pl.cpsign = pl.ref = 0;
-    if (act) pl = load_plane_lut<CP3>(a.lut, ref0, sig0, sign0, cpoff, bp, lbad);
+    pl.isig = pl.isign = pl.iref = pl.icp = 0;
+    pl.tal = false;
+    if (TRAIN) {
+      pl.isig = sig0 + bp * N_SIG;
+      pl.isign = sign0 + bp * N_SIGN;
+      pl.iref = ref0 + bp;
+      pl.icp = cpoff;
+      pl.tal = bp < a.nbp;
+    } else if (act) {
+      pl = load_plane_lut<CP3>(a.lut, ref0, sig0, sign0, cpoff, bp, lbad);
+    }
     if (lbad) status |= ST_RANGE;
     u64 next[2] = {0, 0};
     if (!DEC && bp > 0) scr.plane(bp - 1, next);
     const bool act_sr = act && !(CP3 && bp == msb);
     if (ballot(act_sr)) {
-      sig_pass<DEC, CP3, 0>(s, act_sr, plane, pl, g, a.src, a.dst);
-      ref_pass<DEC>(s, act_sr, plane, pl, g, a.src, a.dst);
+      sig_pass<DEC, CP3, 0, TRAIN>(s, act_sr, plane, pl, g, a.src, a.dst);
+      ref_pass<DEC, TRAIN>(s, act_sr, plane, pl, g, a.src, a.dst);
     }
-    if (CP3) sig_pass<DEC, CP3, 1>(s, act, plane, pl, g, a.src, a.dst);
+    if (CP3) sig_pass<DEC, CP3, 1, TRAIN>(s, act, plane, pl, g, a.src, a.dst);
     if (!DEC) {
       plane[0] = next[0];
       plane[1] = next[1];
@@ -539,9 +573,9 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
   }
 
   const u32 cnt = half ? s.cnt1 : s.cnt0;  // next free slot of this codeblock
-  const bool overflow = good && cnt > (u32)a.stride;
+  const bool overflow = !TRAIN && good && cnt > (u32)a.stride;
   if (overflow) status |= ST_OVERFLOW;
-  if (!DEC) {
+  if (!DEC && !TRAIN) {
     // flush of each lane's open codeword (BPC_PaCo.cu:1020)
     if (good) a.dst[g.region + s.slot] = (int32_t)s.L;
     // -1 in every word past the used slots (BPC_PaCo.cu:1284); a codeblock that was
@@ -549,7 +583,7 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
     if (valid && a.fill)
       for (u32 j = (good ? cnt : 1u) + hl; j < (u32)a.stride; j += 32) a.dst[g.region + j] = -1;
     if (valid && hl == 0 && a.ncw) a.ncw[cb] = good ? (int32_t)(cnt - 1) : 0;
-  } else {
+  } else if (DEC) {
     // Write_Coefficients (BPC_PaCo.cu:46-63): magnitude from the decoded planes
     const bool emit = good && !overflow;
     if (valid) {
@@ -578,7 +612,7 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
       }
     }
   }
-  if (status) atomicOr(status_word, status);
+  if (status && status_word) atomicOr(status_word, status);
 }
 
 
@@ -756,7 +790,97 @@ __global__ __launch_bounds__(256) void queue_pair_scatter_kernel(const QueueArgs
 __global__ void status_fold_kernel(u32* ctl, uint32_t* status) {
   if (threadIdx.x == 0 && ctl[CTL_STATUS]) atomicOr(status, ctl[CTL_STATUS]);
 }
+
+// ---- LUT trainer ------------------------------------------------------------
+// Symbol statistics of a training image: the encoder's modelling in symbol-count
+// mode, every plane in registers (no codeblock is too wide).  The waves of a
+// workgroup share one LDS histogram over the LUT entries; each workgroup writes
+// its histogram to its own partial row, and lut_from_counts_kernel sums the rows.
+template <int CP3>
+__global__ __launch_bounds__(64 * TRAIN_WAVES) void bpc_train_kernel(const TrainArgs a) {
+  extern __shared__ unsigned long long hist[];
+  for (int k = threadIdx.x; k < a.lut_len; k += blockDim.x) hist[k] = 0;
+  __syncthreads();
+  const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * TRAIN_WAVES + (threadIdx.x >> 6)));
+  const int nwaves = (int)gridDim.x * TRAIN_WAVES;
+  void* scr = reinterpret_cast<char*>(a.scratch) + (size_t)wave * scratch_bytes_per_wave(TRAIN_MAXP);
+  HalfDesc d;
+  d.src = a.img;
+  d.dst = nullptr;
+  d.lut = nullptr;
+  d.ncw = nullptr;
+  d.size_x = a.size_x;
+  d.size_y = a.size_y;
+  d.nx = a.nx;
+  d.stride = 1;
+  d.nbp = a.nbp;
+  d.levels = a.levels;
+  d.lut_len = a.lut_len;
+  d.layout = a.layout;
+  d.fill = 0;
+  const int npairs = (a.ncb + 1) / 2;
+  for (int pair = wave; pair < npairs; pair += nwaves) {
+    const int cb = 2 * pair + ((threadIdx.x & 63) >> 5);
+    d.cb = cb < a.ncb ? cb : -1;
+    code_pair<0, CP3, TRAIN_MAXP, 1>(d, a.rows, nullptr, scr, hist);
+  }
+  __syncthreads();
+  unsigned long long* row = a.partial + (size_t)blockIdx.x * a.lut_len;
+  for (int k = threadIdx.x; k < a.lut_len; k += blockDim.x) row[k] = hist[k];
+}
+
+// Sums the partial rows per LUT entry; P = clamp(round(128 * zeros / symbols), 1, 127),
+// 64 for an entry that codes nothing.  Optional outputs: the summed counts.
+__global__ __launch_bounds__(256) void lut_from_counts_kernel(const unsigned long long* partial, int rows_n,
+                                                               int lut_len, int32_t* lut, uint32_t* n0_out,
+                                                               uint32_t* n_out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= lut_len) return;
+  unsigned long long t = 0;
+  for (int b = 0; b < rows_n; ++b) t += partial[(size_t)b * lut_len + k];
+  const unsigned long long n = t >> 32, n0 = t & 0xFFFFFFFFull;
+  if (lut) {
+    int v = 64;
+    if (n) v = min(127, max(1, (int)((n0 * 128 + n / 2) / n)));
+    lut[k] = v;
+  }
+  if (n0_out) n0_out[k] = (uint32_t)n0;
+  if (n_out) n_out[k] = (uint32_t)n;
+}
 }  // namespace
+
+int train_resident_blocks_per_cu(int cpass3, int lds_bytes) {
+  int blocks = 0;
+  if (hipFuncSetAttribute(cpass3 ? (const void*)bpc_train_kernel<1> : (const void*)bpc_train_kernel<0>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes) != hipSuccess)
+    return 0;
+  hipError_t e = cpass3 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, bpc_train_kernel<1>, 64 * TRAIN_WAVES,
+                                                                       lds_bytes)
+                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, bpc_train_kernel<0>, 64 * TRAIN_WAVES,
+                                                                       lds_bytes);
+  return e == hipSuccess ? blocks : 0;
+}
+
+hipError_t launch_train(int cpass3, const TrainArgs& a, int grid_blocks, int32_t* lut, uint32_t* n0, uint32_t* n,
+                        hipStream_t stream) {
+  const size_t lds = (size_t)a.lut_len * sizeof(unsigned long long);
+  if (cpass3) {
+    hipError_t e = hipFuncSetAttribute((const void*)bpc_train_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(bpc_train_kernel<1>, dim3(grid_blocks), dim3(64 * TRAIN_WAVES), lds, stream, a);
+  } else {
+    hipError_t e = hipFuncSetAttribute((const void*)bpc_train_kernel<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(bpc_train_kernel<0>, dim3(grid_blocks), dim3(64 * TRAIN_WAVES), lds, stream, a);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(lut_from_counts_kernel, dim3((a.lut_len + 255) / 256), dim3(256), 0, stream, a.partial,
+                     grid_blocks, a.lut_len, lut, n0, n);
+  return hipGetLastError();
+}
 
 // Device-to-device copy, 16 bytes per lane per iteration (grid-stride).
 __global__ __launch_bounds__(256) void copy_kernel(const int4* __restrict__ src, int4* __restrict__ dst, size_t n16) {

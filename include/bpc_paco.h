@@ -95,6 +95,16 @@ int bpc_lut_free(int32_t* d_lut);
  * 64 where a row was never visited.  nthreads <= 0: all host threads. */
 int bpc_lut_train(const int32_t* img, int size_x, int size_y, const bpc_params* p, int32_t* host_lut, int lut_len,
                   int nthreads);
+/* The same trainer on the device, asynchronous on `stream`: d_img is a training
+ * image of size_x*size_y int32 in p->layout (raster or codeblock-contiguous).
+ * Writes bpc_lut_len(p) entries to each non-null output: d_lut (the table, ready
+ * for bpc_encode_device), d_n0 / d_n (per LUT entry: symbols 0 / all symbols counted).
+ * Codeblocks with a magnitude >= 2^28 are skipped, as by bpc_lut_train.  Uses the
+ * library's per-device scratch (ordered with the codec launches that share it).
+ * No counterpart in the reference, which ships trained tables only
+ * (main_example.cu:219-448 reads them). */
+int bpc_lut_train_device(const int32_t* d_img, int size_x, int size_y, const bpc_params* p, int32_t* d_lut,
+                         uint32_t* d_n0, uint32_t* d_n, void* stream);
 
 /* ---- device-resident codec (no host copies; asynchronous on `stream`) ----------
  * d_coeffs: size_x*size_y int32 in p->layout; d_stream: bpc_stream_words() int32.

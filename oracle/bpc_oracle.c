@@ -160,6 +160,8 @@ typedef struct {
   const int32_t* lut;
   int ref[NLANES], sig[NLANES], sign[NLANES]; /* LUT row pointers at plane bp = 0 */
   int cpoff;                                  /* cp_sig/cp_sign offset (BPC_PaCo.cu:988-989) */
+  uint32_t *n0, *n; /* symbol-count mode (non-NULL): per LUT entry, zeros and all symbols */
+  int count_nbp;    /* symbol-count mode: planes >= this are modelled but not counted */
 } orc_cb;
 
 /* ---- arithmetic coder, BPC_PaCo.cu:261-295 (encoder) ---- */
@@ -183,6 +185,19 @@ static void ac_encode(orc_cb* c, int lane, uint32_t sym, int prob) {
     else
       c->err = ORC_ERR_OVERFLOW;
   }
+}
+
+/* Encoder symbol sink: arithmetic-code sym with probability lut[idx], or, in
+ * symbol-count mode, count it at LUT entry idx instead (no codeword is formed). */
+static void emit(orc_cb* c, int lane, uint32_t sym, int idx, int bp) {
+  if (c->n) {
+    if (bp < c->count_nbp) {
+      c->n[idx]++;
+      if (!sym) c->n0[idx]++;
+    }
+    return;
+  }
+  ac_encode(c, lane, sym, c->lut[idx]);
 }
 
 /* ---- arithmetic coder, BPC_PaCo.cu:297-339 (decoder) ---- */
@@ -277,7 +292,7 @@ static void sig_step(orc_cb* c, int row, int odd, int bp, int cleanup, int decod
     uint32_t sym;
     if (!decode) {
       sym = (*w >> (bp + 1)) & 1u;
-      ac_encode(c, l, sym, c->lut[sbase + ctx]);
+      emit(c, l, sym, sbase + ctx, bp);
     } else {
       sym = ac_decode(c, l, c->lut[sbase + ctx]);
     }
@@ -294,12 +309,11 @@ static void sig_step(orc_cb* c, int row, int odd, int bp, int cleanup, int decod
     uint32_t* w = &c->w[l][ci];
     const uint32_t* n = nb[l];
     const int sctx = orc_sign_context(n[1], n[3], n[4], n[6]); /* (U, L, R, D) */
-    const int pbase = c->sign[l] + bp * N_SIGN + (cleanup ? c->cpoff : 0);
-    const int prob = c->lut[pbase + (sctx >> 1)];
+    const int pidx = c->sign[l] + bp * N_SIGN + (cleanup ? c->cpoff : 0) + (sctx >> 1);
     if (!decode) {
-      ac_encode(c, l, ((*w & 1u) == (uint32_t)(sctx & 1)) ? 0u : 1u, prob);
+      emit(c, l, ((*w & 1u) == (uint32_t)(sctx & 1)) ? 0u : 1u, pidx, bp);
     } else {
-      uint32_t s = ac_decode(c, l, prob);
+      uint32_t s = ac_decode(c, l, c->lut[pidx]);
       s = ((s & 1u) == (uint32_t)(sctx & 1)) ? 0u : 1u;
       *w |= s;
     }
@@ -311,12 +325,12 @@ static void ref_step(orc_cb* c, int row, int odd, int bp, int decode, uint32_t m
   const int ci = 2 * row + odd;
   for (int l = 0; l < NLANES; ++l) {
     uint32_t* w = &c->w[l][ci];
-    const int prob = c->lut[c->ref[l] + bp * N_REF];
+    const int ridx = c->ref[l] + bp * N_REF;
     if (*w & R_BIT) {
       if (!decode) {
-        ac_encode(c, l, (*w >> (bp + 1)) & 1u, prob);
+        emit(c, l, (*w >> (bp + 1)) & 1u, ridx, bp);
       } else {
-        uint32_t s = ac_decode(c, l, prob);
+        uint32_t s = ac_decode(c, l, c->lut[ridx]);
         *w &= ~mask;
         *w |= mask & (((s << 1) + 1u) << bp);
       }
@@ -389,20 +403,13 @@ static void run_planes(orc_cb* c, int msb, int cp3, int decode) {
   }
 }
 
-int orc_encode_cb(const int32_t* img, int size_x, int size_y, int cb, const int32_t* lut, int lut_len,
-                  const orc_params* p, int32_t* region, int32_t* ncw) {
-  int st = check_params(size_x, size_y, p, lut_len);
-  if (st) return st;
-  if (cb < 0 || cb >= (size_x / 64) * (size_y / p->cblock_length)) return ORC_ERR_ARG;
-  orc_cb* c = (orc_cb*)calloc(1, sizeof(orc_cb));
-  if (!c) return ORC_ERR_ARG;
+/* Read_Coefficients (BPC_PaCo.cu:14-44) of codeblock cb into the lanes' words and
+ * its MSB (Find_MSB, :114-161: OR of (TData >> 1) with the C image cleared).
+ * Returns nonzero when a magnitude is >= 2^28 (it would reach the flag bits). */
+static int read_coefficients(orc_cb* c, const int32_t* img, int size_x, int cb, const orc_params* p, int* msb) {
   const int nx = size_x / 64;
   const int tx = cb % nx, ty = cb / nx;
   c->rows = p->cblock_length;
-  c->stride = stride_of(p);
-  c->lut = lut;
-  c->wregion = region;
-  /* Read_Coefficients, BPC_PaCo.cu:14-44 */
   uint32_t m = 0;
   int range_err = 0;
   for (int l = 0; l < NLANES; ++l)
@@ -414,13 +421,27 @@ int orc_encode_cb(const int32_t* img, int size_x, int size_y, int cb, const int3
         uint32_t w = (mag << 1) | (v < 0 ? 1u : 0u);
         if (p->cpass3) w |= C_BIT;
         c->w[l][2 * r + k] = w;
-        m |= mag; /* Find_MSB: OR of (TData >> 1) with the C image cleared, :114-161 */
+        m |= mag;
       }
-  if (range_err) {
+  *msb = msb_of(m);
+  return range_err;
+}
+
+int orc_encode_cb(const int32_t* img, int size_x, int size_y, int cb, const int32_t* lut, int lut_len,
+                  const orc_params* p, int32_t* region, int32_t* ncw) {
+  int st = check_params(size_x, size_y, p, lut_len);
+  if (st) return st;
+  if (cb < 0 || cb >= (size_x / 64) * (size_y / p->cblock_length)) return ORC_ERR_ARG;
+  orc_cb* c = (orc_cb*)calloc(1, sizeof(orc_cb));
+  if (!c) return ORC_ERR_ARG;
+  c->stride = stride_of(p);
+  c->lut = lut;
+  c->wregion = region;
+  int msb;
+  if (read_coefficients(c, img, size_x, cb, p, &msb)) {
     free(c);
     return ORC_ERR_RANGE;
   }
-  const int msb = msb_of(m);
   region[0] = msb; /* Output[WCoordinate] = MSB, BPC_PaCo.cu:1155 */
   if (ncw) *ncw = 0;
   if (msb == 32) { /* all-zero codeblock: nothing coded */
@@ -690,4 +711,57 @@ int orc_lut_load_text(const char* folder, int cpass3, int dwt_levels, int32_t* o
   if (nbp) *nbp = maxbp;
   if (len) *len = total;
   return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* symbol statistics (the input of LUT training; the reference ships trained   */
+/* tables only, main_example.cu:219-448, so the counting rule is this build's) */
+
+int orc_symbol_counts(const int32_t* img, int size_x, int size_y, const orc_params* p, uint32_t* n0, uint32_t* n,
+                      int lut_len, int nthreads) {
+  int st = check_params(size_x, size_y, p, lut_len);
+  if (st) return st;
+  const int ncb = (size_x / 64) * (size_y / p->cblock_length);
+  memset(n0, 0, sizeof(uint32_t) * (size_t)lut_len);
+  memset(n, 0, sizeof(uint32_t) * (size_t)lut_len);
+  int err = ORC_OK;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+#pragma omp parallel
+  {
+    uint32_t* l0 = (uint32_t*)calloc((size_t)lut_len, sizeof(uint32_t));
+    uint32_t* ln = (uint32_t*)calloc((size_t)lut_len, sizeof(uint32_t));
+    orc_cb* c = (orc_cb*)malloc(sizeof(orc_cb));
+    int lerr = (!l0 || !ln || !c) ? ORC_ERR_ARG : ORC_OK;
+#pragma omp for schedule(dynamic, 4)
+    for (int cb = 0; cb < ncb; ++cb) {
+      if (lerr) continue;
+      memset(c, 0, sizeof *c);
+      int msb;
+      /* codeblocks the encoder rejects (a magnitude >= 2^28) or does not code (all zero) count nothing */
+      if (read_coefficients(c, img, size_x, cb, p, &msb) || msb == 32) continue;
+      /* rows of planes >= lut_n_bitplanes are not counted, so no LUT bound applies */
+      (void)setup_lanes(c, size_x, size_y, cb, p, 0x7FFFFFFF, msb);
+      c->n0 = l0;
+      c->n = ln;
+      c->count_nbp = p->lut_n_bitplanes;
+      run_planes(c, msb, p->cpass3, 0);
+    }
+#pragma omp critical
+    {
+      err = merge_err(err, lerr);
+      if (!lerr)
+        for (int k = 0; k < lut_len; ++k) {
+          n0[k] += l0[k];
+          n[k] += ln[k];
+        }
+    }
+    free(l0);
+    free(ln);
+    free(c);
+  }
+  return err;
 }

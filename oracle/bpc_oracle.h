@@ -62,6 +62,14 @@ int orc_encode_list(const int32_t* img, int size_x, int size_y, const int32_t* c
 int orc_decode_list(const int32_t* stream, int size_x, int size_y, const int32_t* cbs, int ncbs,
                     const int32_t* lut, int lut_len, const orc_params* p, int32_t* img, int nthreads);
 
+/* Symbol statistics of an image: the encoder's modelling (passes, scan order,
+ * contexts) with every coded symbol counted at the LUT entry whose probability
+ * would code it, in n[] (all symbols) and n0[] (symbols 0); no codewords are
+ * formed.  Planes >= lut_n_bitplanes are modelled but not counted; codeblocks
+ * with a magnitude >= 2^28 count nothing.  Arrays hold lut_len entries. */
+int orc_symbol_counts(const int32_t* img, int size_x, int size_y, const orc_params* p, uint32_t* n0, uint32_t* n,
+                      int lut_len, int nthreads);
+
 /* ---- primitives exposed for unit tests ---------------------------------- */
 /* Significance context: number of significant 8-neighbours (BPC_PaCo.cu:164-172). */
 int orc_sig_context(const uint32_t nb[8]);

@@ -51,6 +51,7 @@ def lib():
         L.orc_decode_list.argtypes = [P, i32, i32, P, i32, P, i32, PP, P, i32]
         L.orc_encode_cb.argtypes = [P, i32, i32, i32, P, i32, PP, P, P]
         L.orc_decode_cb.argtypes = [P, i32, i32, i32, P, i32, PP, P]
+        L.orc_symbol_counts.argtypes = [P, i32, i32, PP, P, P, i32, i32]
         L.orc_sig_context.argtypes = [P]
         L.orc_sign_context_hv.argtypes = [i32, i32]
         L.orc_sign_context.argtypes = [ctypes.c_uint32] * 4
@@ -125,6 +126,18 @@ def decode_list(stream, size_x, size_y, cbs, lut, p, nthreads=0):
     st = lib().orc_decode_list(_ptr(stream), size_x, size_y, _ptr(cbs), cbs.size, _ptr(lut), lut.size,
                                ctypes.byref(p), _ptr(img), nthreads)
     return st, img
+
+
+def symbol_counts(img, p, nthreads=0):
+    """Per LUT entry: (status, n0, n) = symbols 0 and all symbols the encoder codes
+    with that entry's probability (planes >= lut_n_bitplanes not counted)."""
+    img = np.ascontiguousarray(img, dtype=np.int32)
+    Y, X = img.shape
+    n = lut_len(p)
+    n0 = np.zeros(n, dtype=np.uint32)
+    nn = np.zeros(n, dtype=np.uint32)
+    st = lib().orc_symbol_counts(_ptr(img), X, Y, ctypes.byref(p), _ptr(n0), _ptr(nn), n, nthreads)
+    return st, n0, nn
 
 
 def sig_context(nb):
