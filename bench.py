@@ -399,6 +399,50 @@ def bench_large(args, check=True):
     return out
 
 
+def bench_container(args, d_cs, d_ncw, ncb, stride, stream):
+    """Compact container of the plain workload's codestream: index + pack (regions ->
+    MSB bytes, offsets, uint16 codewords) and unpack (back to -1-filled regions), with
+    the round trip checked.  HBM-bound; algorithmic bytes per codeblock: pack reads
+    the MSB, count and used words (4 * (2 + n)) and writes 1 + 8 + 2n; unpack reads
+    1 + 8 + 2n and writes the whole region (4 * stride)."""
+    msb = bpc.DeviceBuffer(ncb, np.uint8)
+    off = bpc.DeviceBuffer(ncb + 1, np.uint64)
+    bpc.compact_index(d_cs, ncb, stride, msb, off, d_ncw=d_ncw, stream=stream)
+    total = int(bpc.download_ptr(off.ptr.value + 8 * ncb, 1, np.uint64, stream)[0])
+    payload = bpc.DeviceBuffer(max(total, 1), np.uint16)
+    back = bpc.DeviceBuffer(ncb * stride)
+    c = bpc.Compact(msb, off, payload, ncb, total)
+    ev = [bpc.Event() for _ in range(3)]
+    tp, tu = [], []
+    for k in range(max(1, args.warmup) + max(5, args.steps)):
+        ev[0].record(stream)
+        bpc.compact_index(d_cs, ncb, stride, msb, off, d_ncw=d_ncw, stream=stream)
+        bpc.compact_pack_into(d_cs, ncb, stride, off, payload, total, stream=stream)
+        ev[1].record(stream)
+        bpc.compact_unpack(c, back, stride, stream=stream)
+        ev[2].record(stream)
+        if k >= max(1, args.warmup):
+            tp.append(ev[0].elapsed_ms(ev[1]))
+            tu.append(ev[1].elapsed_ms(ev[2]))
+    if bpc.status_take(None, stream) != 0:
+        raise RuntimeError("container status: %s" % bpc.lib().bpc_last_error().decode())
+    exact = bool(np.array_equal(back.download(stream=stream), d_cs.download(stream=stream)))
+    if not exact:
+        raise RuntimeError("compact round trip differs")
+    pack_ms, unpack_ms = float(np.median(tp)), float(np.median(tu))
+    pack_bytes = 4.0 * (2 * ncb + total) + 9.0 * ncb + 2.0 * total
+    unpack_bytes = 9.0 * ncb + 2.0 * total + 4.0 * stride * ncb
+    for b in (payload, back, msb, off):
+        b.free()
+    return {"workload": "compact container of the plain_4096_3pass codestream",
+            "compact_bytes": c.nbytes, "reference_layout_bytes": 4 * stride * ncb,
+            "compact_fraction": round(c.nbytes / (4.0 * stride * ncb), 4),
+            "pack_ms": round(pack_ms, 4), "unpack_ms": round(unpack_ms, 4),
+            "pack_gbs": round(pack_bytes / pack_ms / 1e6, 1), "unpack_gbs": round(unpack_bytes / unpack_ms / 1e6, 1),
+            "unpack_hbm_frac": round(unpack_bytes / unpack_ms / 1e6 / HBM_PEAK_GBS, 4),
+            "round_trip_exact": exact}
+
+
 def bench_train(args):
     """LUT training on the device (symbol statistics of the encoder's modelling, no
     coding) for the plain workload's training image: 4096x4096, 3 passes, 16 LUT
@@ -445,8 +489,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle baseline and parity check")
-    ap.add_argument("--workloads", default="plain,batched,served,large,train",
-                    help="comma list of plain, batched, served, large, train (plain always runs: it gives `value`)")
+    ap.add_argument("--workloads", default="plain,batched,served,large,train,container",
+                    help="comma list of plain, batched, served, large, train, container "
+                         "(plain always runs: it gives `value`)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB 5 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -572,10 +617,12 @@ def main():
         out["bit_exact_vs_oracle"] = bool(np.array_equal(gstream, ostream))
         if not out["bit_exact_vs_oracle"]:
             raise RuntimeError("GPU codestream differs from the oracle")
+    wl = args.workloads.split(",")
+    if "container" in wl:
+        out["container"] = bench_container(args, d_cs, d_ncw, ncb, p.stride(), stream)
     for b in (d_in, d_cs, d_out, d_ncw, d_status):
         b.free()
     lut.free()
-    wl = args.workloads.split(",")
     if "batched" in wl:
         out["batched"] = bench_batched(args, check=not args.no_cpu)
     if "served" in wl:

@@ -132,6 +132,12 @@ def lib():
             "bpc_lut_free": (i32, [P]),
             "bpc_lut_train": (i32, [P, i32, i32, PP, P, i32, i32]),
             "bpc_lut_train_device": (i32, [P, i32, i32, PP, P, P, P, P]),
+            "bpc_compact_index_device": (i32, [P, ctypes.c_int64, i32, P, P, P, P, P]),
+            "bpc_compact_pack_device": (i32, [P, ctypes.c_int64, i32, P, P, ctypes.c_uint64, P, P]),
+            "bpc_compact_unpack_device": (i32, [P, P, P, ctypes.c_uint64, ctypes.c_int64, i32, P, P, P]),
+            "bpc_cb_file_order": (i32, [i32, i32, i32, i32, P, i32]),
+            "bpc_bin_export": (i32, [ctypes.c_char_p, P, P, P, i32, i32, PP]),
+            "bpc_bin_import": (i32, [ctypes.c_char_p, i32, i32, PP, P]),
             "bpc_encode_device": (i32, [P, P, i32, i32, P, i32, PP, i32, P, P, P]),
             "bpc_decode_device": (i32, [P, P, i32, i32, P, i32, PP, P, P]),
             "bpc_status_take": (i32, [P, P]),
@@ -562,4 +568,81 @@ def download_ptr(ptr, n, dtype=np.int32, stream=None):
     """Copies n elements from a raw device pointer to a new host array."""
     out = np.empty(n, dtype=dtype)
     _check(lib().bpc_copy_d2h(_ptr(out), ctypes.c_void_p(ptr), out.nbytes, _sp(stream)))
+    return out
+
+
+# ---- compact codestream container / per-codeblock files -------------------------------
+
+class Compact:
+    """A compact codestream on the device: msb (uint8[ncb]), offsets (uint64[ncb+1],
+    in codewords) and payload (uint16[offsets[ncb]])."""
+
+    def __init__(self, msb, offsets, payload, ncb, total):
+        self.msb, self.offsets, self.payload = msb, offsets, payload
+        self.ncb, self.total = ncb, total
+
+    @property
+    def nbytes(self):
+        return self.ncb + 8 * (self.ncb + 1) + 2 * self.total
+
+    def download(self, stream=None):
+        return (self.msb.download(stream=stream), self.offsets.download(stream=stream),
+                self.payload.download(self.total, stream=stream))
+
+    def free(self):
+        for b in (self.msb, self.offsets, self.payload):
+            b.free()
+
+
+def compact_index(d_stream, ncb, stride, d_msb, d_offsets, d_ncw=None, d_status=None, stream=None):
+    _check(lib().bpc_compact_index_device(_dptr(d_stream), ncb, stride, _dptr(d_ncw), _dptr(d_msb), _dptr(d_offsets),
+                                          _dptr(d_status), _sp(stream)))
+
+
+def compact_pack_into(d_stream, ncb, stride, d_offsets, d_payload, cap, d_status=None, stream=None):
+    _check(lib().bpc_compact_pack_device(_dptr(d_stream), ncb, stride, _dptr(d_offsets), _dptr(d_payload), cap,
+                                         _dptr(d_status), _sp(stream)))
+
+
+def compact_pack(d_stream, ncb, stride, d_ncw=None, d_status=None, stream=None):
+    """Reference-layout regions on the device -> Compact (synchronizes once to size
+    the payload)."""
+    msb = DeviceBuffer(ncb, np.uint8)
+    off = DeviceBuffer(ncb + 1, np.uint64)
+    compact_index(d_stream, ncb, stride, msb, off, d_ncw, d_status, stream)
+    total = int(download_ptr(off.ptr.value + 8 * ncb, 1, np.uint64, stream)[0])
+    payload = DeviceBuffer(max(total, 1), np.uint16)
+    compact_pack_into(d_stream, ncb, stride, off, payload, total, d_status, stream)
+    return Compact(msb, off, payload, ncb, total)
+
+
+def compact_unpack(c, d_stream, stride, d_status=None, stream=None):
+    """Compact -> reference-layout regions (MSB, codewords, -1 fill) in d_stream."""
+    _check(lib().bpc_compact_unpack_device(_dptr(c.msb), _dptr(c.offsets), _dptr(c.payload), c.total, c.ncb, stride,
+                                           _dptr(d_stream), _dptr(d_status), _sp(stream)))
+
+
+def cb_file_order(size_x, size_y, dwt_levels=5, cblock_length=64):
+    """Tile index of every codeblock file k (main_example.cu:71-179's order)."""
+    n = (size_x // 64) * (size_y // cblock_length)
+    out = np.zeros(max(n, 1), dtype=np.int32)
+    k = lib().bpc_cb_file_order(size_x, size_y, dwt_levels, cblock_length, _ptr(out), out.size)
+    if k < 0:
+        raise BPCError(k, lib().bpc_last_error().decode(errors="replace"))
+    return out[:k].copy()
+
+
+def bin_export(folder, msb, offsets, payload, size_x, size_y, p):
+    msb = np.ascontiguousarray(msb, dtype=np.uint8)
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+    payload = np.ascontiguousarray(payload, dtype=np.uint16)
+    if payload.size == 0:
+        payload = np.zeros(1, np.uint16)
+    _check(lib().bpc_bin_export(folder.encode(), _ptr(msb), _ptr(offsets), _ptr(payload), size_x, size_y,
+                                ctypes.byref(p)))
+
+
+def bin_import(folder, size_x, size_y, p):
+    out = np.zeros(stream_words(size_x, size_y, p), dtype=np.int32)
+    _check(lib().bpc_bin_import(folder.encode(), size_x, size_y, ctypes.byref(p), _ptr(out)))
     return out

@@ -1017,3 +1017,158 @@ int bpc_batch_run(bpc_batch* b, int direction, const bpc_image* images, int n_im
 }
 
 }  // extern "C"
+
+// ---- compact codestream container and per-codeblock files ------------------
+
+extern "C" {
+
+int bpc_compact_index_device(const int32_t* d_stream, int64_t ncb, int stride_words, const int32_t* d_ncw,
+                             uint8_t* d_msb, uint64_t* d_offsets, uint32_t* d_status, void* stream) {
+  if (ncb < 0 || stride_words < 2) return fail(BPC_ERR_ARG, "bad codeblock count or stride");
+  if (ncb == 0) return BPC_OK;
+  if (!d_stream || !d_msb || !d_offsets) return fail(BPC_ERR_ARG, "null buffer");
+  if (ncb > ((int64_t)1 << 31)) return fail(BPC_ERR_ARG, "too many codeblocks");
+  if (!d_status) {
+    int st = default_status(&d_status);
+    if (st) return st;
+  }
+  hipError_t e = bpc::launch_compact_index(d_stream, ncb, stride_words, d_ncw, d_msb,
+                                           reinterpret_cast<unsigned long long*>(d_offsets), d_status,
+                                           (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "compact index launch");
+  return BPC_OK;
+}
+
+int bpc_compact_pack_device(const int32_t* d_stream, int64_t ncb, int stride_words, const uint64_t* d_offsets,
+                            uint16_t* d_payload, uint64_t payload_cap, uint32_t* d_status, void* stream) {
+  if (ncb < 0 || stride_words < 2) return fail(BPC_ERR_ARG, "bad codeblock count or stride");
+  if (ncb == 0) return BPC_OK;
+  if (!d_stream || !d_offsets || (!d_payload && payload_cap)) return fail(BPC_ERR_ARG, "null buffer");
+  if (!d_status) {
+    int st = default_status(&d_status);
+    if (st) return st;
+  }
+  hipError_t e = bpc::launch_compact_pack(d_stream, ncb, stride_words,
+                                          reinterpret_cast<const unsigned long long*>(d_offsets), d_payload,
+                                          payload_cap, d_status, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "compact pack launch");
+  return BPC_OK;
+}
+
+int bpc_compact_unpack_device(const uint8_t* d_msb, const uint64_t* d_offsets, const uint16_t* d_payload,
+                              uint64_t payload_len, int64_t ncb, int stride_words, int32_t* d_stream,
+                              uint32_t* d_status, void* stream) {
+  if (ncb < 0 || stride_words < 2) return fail(BPC_ERR_ARG, "bad codeblock count or stride");
+  if (ncb == 0) return BPC_OK;
+  if (!d_msb || !d_offsets || !d_stream || (!d_payload && payload_len)) return fail(BPC_ERR_ARG, "null buffer");
+  if (!d_status) {
+    int st = default_status(&d_status);
+    if (st) return st;
+  }
+  hipError_t e = bpc::launch_compact_unpack(d_msb, reinterpret_cast<const unsigned long long*>(d_offsets), d_payload,
+                                            payload_len, ncb, stride_words, d_stream, d_status, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "compact unpack launch");
+  return BPC_OK;
+}
+
+// load_Data's codeblock file order (main_example.cu:71-179): the LL band, then per
+// level from the coarsest the HL, LH and HH bands, each in row-major codeblock steps.
+int bpc_cb_file_order(int size_x, int size_y, int dwt_levels, int cblock_length, int32_t* tile_of_file, int cap) {
+  if (size_x <= 0 || size_y <= 0 || size_x % 64 || cblock_length < 2 || cblock_length > 64 ||
+      size_y % cblock_length || dwt_levels < 0 || dwt_levels > 16 || !tile_of_file)
+    return fail(BPC_ERR_ARG, "bad image geometry");
+  const int nx = size_x / 64;
+  const int ncb = nx * (size_y / cblock_length);
+  if (cap < ncb) return fail(BPC_ERR_ARG, "capacity %d < %d codeblocks", cap, ncb);
+  std::vector<char> seen(ncb, 0);
+  int k = 0;
+  auto band = [&](int x0, int y0, int sx, int sy) -> bool {
+    if (sx % 64 || sy % cblock_length) return false;
+    for (int j = 0; j < sy; j += cblock_length)
+      for (int i = 0; i < sx; i += 64) {
+        const int t = ((y0 + j) / cblock_length) * nx + (x0 + i) / 64;
+        if (k >= ncb || seen[t]) return false;
+        seen[t] = 1;
+        tile_of_file[k++] = t;
+      }
+    return true;
+  };
+  bool ok = band(0, 0, size_x >> dwt_levels, size_y >> dwt_levels);
+  for (int d = dwt_levels; ok && d > 0; --d) {
+    const int sx = size_x >> d, sy = size_y >> d;
+    ok = band(sx, 0, sx, sy) && band(0, sy, sx, sy) && band(sx, sy, sx, sy);
+  }
+  if (!ok || k != ncb)
+    return fail(BPC_ERR_ARG, "%dx%d with %d levels: subbands are not whole 64x%d codeblocks (no file order)", size_x,
+                size_y, dwt_levels, cblock_length);
+  return k;
+}
+
+// One file per codeblock, named by its index in the file order: the MSB and the
+// codewords as big-endian 16-bit values (what load_CStream, main_example.cu:51-68,
+// reads back into the reference layout, every word past the end reading as -1).
+int bpc_bin_export(const char* folder, const uint8_t* msb, const uint64_t* offsets, const uint16_t* payload,
+                   int size_x, int size_y, const bpc_params* p) {
+  int st = check_params(p);
+  if (st) return st;
+  if (!folder || !msb || !offsets) return fail(BPC_ERR_ARG, "null argument");
+  const int ncb = (size_x / 64) * (size_y / p->cblock_length);
+  std::vector<int32_t> order(ncb > 0 ? ncb : 1);
+  int n = bpc_cb_file_order(size_x, size_y, p->dwt_levels, p->cblock_length, order.data(), ncb);
+  if (n < 0) return n;
+  std::string dir(folder);
+  if (!dir.empty() && dir.back() != '/') dir += '/';
+  std::vector<uint8_t> buf;
+  for (int k = 0; k < n; ++k) {
+    const int t = order[k];
+    const uint64_t o = offsets[t], e = offsets[t + 1];
+    if (e < o || e - o > (uint64_t)stride_of(p) - 1) return fail(BPC_ERR_ARG, "codeblock %d: bad offsets", t);
+    buf.resize(2 * (1 + (e - o)));
+    buf[0] = 0;
+    buf[1] = msb[t];
+    for (uint64_t j = o; j < e; ++j) {
+      buf[2 * (1 + j - o)] = (uint8_t)(payload[j] >> 8);
+      buf[2 * (1 + j - o) + 1] = (uint8_t)(payload[j] & 0xFF);
+    }
+    const std::string path = dir + std::to_string(k) + ".bin";
+    FILE* f = fopen(path.c_str(), "wb");
+    if (!f) return fail(BPC_ERR_ARG, "cannot write %s", path.c_str());
+    const size_t w = fwrite(buf.data(), 1, buf.size(), f);
+    const int c = fclose(f);
+    if (w != buf.size() || c != 0) return fail(BPC_ERR_ARG, "short write to %s", path.c_str());
+  }
+  return BPC_OK;
+}
+
+// load_Data with CStreamDataLoad (main_example.cu:71-179, 51-68): file k fills the
+// region of tile order[k] with stride big-endian 16-bit words, -1 past the end.
+int bpc_bin_import(const char* folder, int size_x, int size_y, const bpc_params* p, int32_t* host_stream) {
+  int st = check_params(p);
+  if (st) return st;
+  if (!folder || !host_stream) return fail(BPC_ERR_ARG, "null argument");
+  const int ncb = (size_x / 64) * (size_y / p->cblock_length);
+  std::vector<int32_t> order(ncb > 0 ? ncb : 1);
+  int n = bpc_cb_file_order(size_x, size_y, p->dwt_levels, p->cblock_length, order.data(), ncb);
+  if (n < 0) return n;
+  std::string dir(folder);
+  if (!dir.empty() && dir.back() != '/') dir += '/';
+  const int stride = stride_of(p);
+  std::vector<uint8_t> buf((size_t)2 * stride);
+  for (int k = 0; k < n; ++k) {
+    const std::string path = dir + std::to_string(k) + ".bin";
+    FILE* f = fopen(path.c_str(), "rb");
+    if (!f) return fail(BPC_ERR_ARG, "cannot read %s", path.c_str());
+    const size_t got = fread(buf.data(), 1, buf.size(), f);
+    fclose(f);
+    int32_t* region = host_stream + (int64_t)order[k] * stride;
+    for (int i = 0; i < stride; ++i) {
+      const size_t b = 2 * (size_t)i;
+      const uint32_t hi = b < got ? buf[b] : 0xFFFFFFFFu;
+      const uint32_t lo = b + 1 < got ? buf[b + 1] : 0xFFFFFFFFu;
+      region[i] = (int32_t)((hi << 8) | lo);
+    }
+  }
+  return BPC_OK;
+}
+
+}  // extern "C"

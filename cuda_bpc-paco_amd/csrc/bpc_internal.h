@@ -105,6 +105,15 @@ int train_resident_blocks_per_cu(int cpass3, int lds_bytes);
 // counts) from the partial rows.
 hipError_t launch_train(int cpass3, const TrainArgs& a, int grid_blocks, int32_t* lut, uint32_t* n0, uint32_t* n,
                         hipStream_t stream);
+// Compact container (bpc_container.hip): index (MSB bytes, codeword counts, scan
+// into offsets[0..ncb]), pack (regions -> payload), unpack (payload -> regions).
+hipError_t launch_compact_index(const int32_t* stream, long long ncb, int stride, const int32_t* ncw, uint8_t* msb,
+                                unsigned long long* offsets, uint32_t* status, hipStream_t s);
+hipError_t launch_compact_pack(const int32_t* stream, long long ncb, int stride, const unsigned long long* offsets,
+                               uint16_t* payload, unsigned long long cap, uint32_t* status, hipStream_t s);
+hipError_t launch_compact_unpack(const uint8_t* msb, const unsigned long long* offsets, const uint16_t* payload,
+                                 unsigned long long cap, long long ncb, int stride, int32_t* stream, uint32_t* status,
+                                 hipStream_t s);
 // 16-byte aligned device-to-device copy of a multiple of 16 bytes.
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
 

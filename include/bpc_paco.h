@@ -121,6 +121,43 @@ int bpc_decode_device(const int32_t* d_stream, int32_t* d_coeffs, int size_x, in
 /* Synchronizes stream, maps the status bits to a BPC_* code and clears them. */
 int bpc_status_take(uint32_t* d_status, void* stream);
 
+/* ---- compact codestream container (no counterpart in the reference) -------------
+ * The reference layout gives every codeblock a fixed region of int32 words
+ * [MSB, codewords..., -1...] (BPC_PaCo.cu:1130,1147-1157,1284).  The compact form
+ * keeps per codeblock one MSB byte (255 marks a malformed MSB word) and its
+ * codewords as uint16 at payload[offsets[cb] .. offsets[cb+1]), offsets[0] = 0.
+ * All three calls are asynchronous on `stream`; errors go to d_status
+ * (nullable: the library's status word), read with bpc_status_take.
+ *
+ * Index: d_msb[ncb] and d_offsets[ncb+1].  Codeword counts come from d_ncw (the
+ * encoder's per-codeblock counts) or, when d_ncw is NULL, from the -1 fill: the
+ * words before the first -1 after the MSB (regions must then be -1-filled). */
+int bpc_compact_index_device(const int32_t* d_stream, int64_t ncb, int stride_words, const int32_t* d_ncw,
+                             uint8_t* d_msb, uint64_t* d_offsets, uint32_t* d_status, void* stream);
+/* Pack: codewords of every region into d_payload (payload_cap uint16 entries;
+ * offsets[ncb] of them are written). */
+int bpc_compact_pack_device(const int32_t* d_stream, int64_t ncb, int stride_words, const uint64_t* d_offsets,
+                            uint16_t* d_payload, uint64_t payload_cap, uint32_t* d_status, void* stream);
+/* Unpack: back to the reference layout (MSB word, codewords, -1 fill); a codeblock
+ * whose codewords do not fit its region keeps only its MSB (BPC_ERR_OVERFLOW). */
+int bpc_compact_unpack_device(const uint8_t* d_msb, const uint64_t* d_offsets, const uint16_t* d_payload,
+                              uint64_t payload_len, int64_t ncb, int stride_words, int32_t* d_stream,
+                              uint32_t* d_status, void* stream);
+
+/* ---- per-codeblock codestream files (main_example.cu:51-179) -------------------
+ * File k.bin belongs to tile tile_of_file[k] in load_Data's order (LL band, then
+ * per level from the coarsest HL, LH, HH, each row-major).  Returns the number of
+ * files, or BPC_ERR_ARG when a subband is not a whole number of codeblocks. */
+int bpc_cb_file_order(int size_x, int size_y, int dwt_levels, int cblock_length, int32_t* tile_of_file, int cap);
+/* Writes folder/k.bin for every codeblock from a host copy of the compact form:
+ * the MSB and the codewords as big-endian 16-bit values (2*(1+n) bytes). */
+int bpc_bin_export(const char* folder, const uint8_t* msb, const uint64_t* offsets, const uint16_t* payload,
+                   int size_x, int size_y, const bpc_params* p);
+/* load_Data with CStreamDataLoad (main_example.cu:71-179) and load_CStream (:51-68):
+ * fills every region of host_stream (bpc_stream_words int32) from folder/k.bin,
+ * big-endian 16-bit words, -1 past the end of each file. */
+int bpc_bin_import(const char* folder, int size_x, int size_y, const bpc_params* p, int32_t* host_stream);
+
 /* ---- batched codec (many images of uneven size in one persistent launch) --------
  * The codeblocks of all images go through one device work queue: sorted by MSB
  * inside each image and paired there (one wave64 codes two codeblocks of one

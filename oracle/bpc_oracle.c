@@ -765,3 +765,50 @@ int orc_symbol_counts(const int32_t* img, int size_x, int size_y, const orc_para
   }
   return err;
 }
+
+/* ------------------------------------------------------------------------- */
+/* per-codeblock codestream files (main_example.cu:51-179)                    */
+
+/* load_Data's enumeration (main_example.cu:71-179): file k of the LL band first,
+ * then for each level from the coarsest the HL (x = sx, y = 0), LH (x = 0, y = sy)
+ * and HH (x = sx, y = sy) bands, each scanned row by row in codeblock steps.
+ * Returns the number of files, or ORC_ERR_ARG when a band is not a whole number
+ * of codeblocks (the enumeration would then visit a tile twice). */
+int orc_cb_file_order(int size_x, int size_y, int dwt_levels, int cblock_length, int32_t* tile_of_file, int cap) {
+  if (size_x <= 0 || size_y <= 0 || dwt_levels < 0 || cblock_length <= 0) return ORC_ERR_ARG;
+  const int nx = size_x / 64;
+  int k = 0;
+  /* LL band */
+  {
+    const int sx = size_x >> dwt_levels, sy = size_y >> dwt_levels;
+    if (sx % 64 || sy % cblock_length) return ORC_ERR_ARG;
+    for (int j = 0; j < sy; j += cblock_length)
+      for (int i = 0; i < sx; i += 64) {
+        if (k >= cap) return ORC_ERR_ARG;
+        tile_of_file[k++] = (j / cblock_length) * nx + i / 64;
+      }
+  }
+  for (int d = dwt_levels; d > 0; --d) {
+    const int sx = size_x >> d, sy = size_y >> d;
+    if (sx % 64 || sy % cblock_length) return ORC_ERR_ARG;
+    const int x0[3] = {sx, 0, sx}, y0[3] = {0, sy, sy};
+    for (int b = 0; b < 3; ++b)
+      for (int j = 0; j < sy; j += cblock_length)
+        for (int i = 0; i < sx; i += 64) {
+          if (k >= cap) return ORC_ERR_ARG;
+          tile_of_file[k++] = ((y0[b] + j) / cblock_length) * nx + (x0[b] + i) / 64;
+        }
+  }
+  return k;
+}
+
+/* load_CStream (main_example.cu:51-68): `words` big-endian 16-bit values; every
+ * value past the end of the file reads as -1 (fgetc's EOF in both bytes). */
+void orc_load_cstream(const uint8_t* bytes, size_t nbytes, int32_t* region, int words) {
+  for (int i = 0; i < words; ++i) {
+    const size_t b = 2 * (size_t)i;
+    const int hi = b < nbytes ? bytes[b] : -1;
+    const int lo = b + 1 < nbytes ? bytes[b + 1] : -1;
+    region[i] = (int32_t)(((uint32_t)hi << 8) | (uint32_t)lo);
+  }
+}

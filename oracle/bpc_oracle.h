@@ -12,6 +12,7 @@
 #ifndef BPC_ORACLE_H
 #define BPC_ORACLE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -69,6 +70,14 @@ int orc_decode_list(const int32_t* stream, int size_x, int size_y, const int32_t
  * with a magnitude >= 2^28 count nothing.  Arrays hold lut_len entries. */
 int orc_symbol_counts(const int32_t* img, int size_x, int size_y, const orc_params* p, uint32_t* n0, uint32_t* n,
                       int lut_len, int nthreads);
+
+/* Codeblock file order of load_Data (main_example.cu:71-179): tile_of_file[k] =
+ * row-major tile index of file k.bin / k.txt.  Returns the file count, or
+ * ORC_ERR_ARG when a band is not a whole number of 64 x cblock_length codeblocks. */
+int orc_cb_file_order(int size_x, int size_y, int dwt_levels, int cblock_length, int32_t* tile_of_file, int cap);
+/* load_CStream (main_example.cu:51-68): words big-endian 16-bit values from a file's
+ * bytes into a region; values past the end of the file read as -1. */
+void orc_load_cstream(const uint8_t* bytes, size_t nbytes, int32_t* region, int words);
 
 /* ---- primitives exposed for unit tests ---------------------------------- */
 /* Significance context: number of significant 8-neighbours (BPC_PaCo.cu:164-172). */

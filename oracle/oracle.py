@@ -52,6 +52,8 @@ def lib():
         L.orc_encode_cb.argtypes = [P, i32, i32, i32, P, i32, PP, P, P]
         L.orc_decode_cb.argtypes = [P, i32, i32, i32, P, i32, PP, P]
         L.orc_symbol_counts.argtypes = [P, i32, i32, PP, P, P, i32, i32]
+        L.orc_cb_file_order.argtypes = [i32, i32, i32, i32, P, i32]
+        L.orc_load_cstream.argtypes = [P, ctypes.c_size_t, P, i32]
         L.orc_sig_context.argtypes = [P]
         L.orc_sign_context_hv.argtypes = [i32, i32]
         L.orc_sign_context.argtypes = [ctypes.c_uint32] * 4
@@ -138,6 +140,22 @@ def symbol_counts(img, p, nthreads=0):
     nn = np.zeros(n, dtype=np.uint32)
     st = lib().orc_symbol_counts(_ptr(img), X, Y, ctypes.byref(p), _ptr(n0), _ptr(nn), n, nthreads)
     return st, n0, nn
+
+
+def cb_file_order(size_x, size_y, dwt_levels, cblock_length=64):
+    """Tile index of each codeblock file k (load_Data's order); None if not a bijection."""
+    n = (size_x // 64) * (size_y // cblock_length)
+    out = np.zeros(n + 1, dtype=np.int32)
+    k = lib().orc_cb_file_order(size_x, size_y, dwt_levels, cblock_length, _ptr(out), out.size)
+    return None if k < 0 else out[:k].copy()
+
+
+def load_cstream(data, words):
+    """load_CStream of one file's bytes into a region of `words` int32."""
+    buf = np.frombuffer(bytes(data), dtype=np.uint8) if len(data) else np.zeros(1, np.uint8)
+    out = np.zeros(words, dtype=np.int32)
+    lib().orc_load_cstream(_ptr(buf), len(data), _ptr(out), words)
+    return out
 
 
 def sig_context(nb):

@@ -1,0 +1,111 @@
+"""Compact codestream container on the device: index / pack / unpack against the
+compact form of the oracle's codestream, exact reference-layout round trip, and
+per-codeblock files written from the device form."""
+import numpy as np
+import pytest
+
+import synth
+from test_container_host import compact_of
+
+pytestmark = pytest.mark.gpu
+
+
+def encoded(bpc, O, size, cp3, nbp, seed, rows=64, stride=0, mag_bits=14, zero_cb=True):
+    img = synth.synth_image(size, size, 5, seed, mag_bits=mag_bits)
+    if zero_cb:
+        img[0:rows, 64:128] = 0
+    p = bpc.params(cpass3=cp3, cblock_length=rows, lut_n_bitplanes=nbp, cb_stride_words=stride)
+    lut_arr = bpc.train_lut(synth.synth_image(size, size, 5, seed + 1, mag_bits=mag_bits), p)
+    st, ostream, oncw = O.encode(img, lut_arr, O.params(cpass3=cp3, cblock_length=rows, lut_n_bitplanes=nbp,
+                                                        cb_stride_words=stride))
+    assert st == 0
+    return img, p, lut_arr, ostream, oncw
+
+
+@pytest.mark.parametrize("size,cp3,nbp,rows,stride,derive", [
+    (2048, 0, 15, 64, 0, False), (2048, 0, 15, 64, 0, True), (1024, 1, 16, 32, 0, True),
+    (1024, 1, 15, 64, 6000, True), (4096, 1, 16, 64, 0, False)])
+def test_pack_unpack_matches_oracle(bpc, O, size, cp3, nbp, rows, stride, derive):
+    print("container", size, cp3, nbp, rows, stride, derive, flush=True)
+    img, p, lut_arr, ostream, oncw = encoded(bpc, O, size, cp3, nbp, 140 + size // 1024 + cp3, rows, stride,
+                                             mag_bits=16 if nbp >= 16 else 14)
+    ncb = p.n_codeblocks(size, size)
+    sw = p.stride()
+    lut = bpc.Lut.from_array(lut_arr)
+    d_img = bpc.DeviceBuffer.from_array(img)
+    d_cs = bpc.DeviceBuffer(ncb * sw)
+    d_ncw = bpc.DeviceBuffer(ncb)
+    d_back = bpc.DeviceBuffer(ncb * sw)
+    d_rec = bpc.DeviceBuffer(size * size)
+    try:
+        bpc.encode_device(d_img, d_cs, size, size, lut, p, 1, d_ncw)
+        assert bpc.status_take() == 0
+        c = bpc.compact_pack(d_cs, ncb, sw, d_ncw=None if derive else d_ncw)
+        assert bpc.status_take() == 0
+        msb, offsets, payload = c.download()
+        emsb, eoff, epay = compact_of(ostream, oncw, sw)
+        assert np.array_equal(msb, emsb)
+        assert np.array_equal(offsets, eoff)
+        assert np.array_equal(payload, epay)
+        assert c.nbytes < 0.6 * ncb * sw * 4  # codewords as 16 bits, no fill
+        d_back.fill_bytes(0x5A)
+        bpc.compact_unpack(c, d_back, sw)
+        assert bpc.status_take() == 0
+        assert np.array_equal(d_back.download(), ostream)
+        bpc.decode_device(d_back, d_rec, size, size, lut, p)
+        assert bpc.status_take() == 0
+        assert np.array_equal(d_rec.download().reshape(size, size), img)
+        c.free()
+    finally:
+        for b in (d_img, d_cs, d_ncw, d_back, d_rec):
+            b.free()
+        lut.free()
+
+
+def test_unpack_rejects_short_regions(bpc, O):
+    """Codewords that do not fit the target regions: overflow status, MSB kept, -1 fill."""
+    img, p, lut_arr, ostream, oncw = encoded(bpc, O, 512, 0, 15, 160)
+    ncb, sw = p.n_codeblocks(512, 512), p.stride()
+    d_cs = bpc.DeviceBuffer.from_array(ostream)
+    c = bpc.compact_pack(d_cs, ncb, sw)
+    assert bpc.status_take() == 0
+    small = 64  # every coded codeblock here has more than 63 codewords
+    assert (oncw[oncw > 0] > small - 1).all()
+    d_small = bpc.DeviceBuffer(ncb * small)
+    bpc.compact_unpack(c, d_small, small)
+    assert bpc.status_take() == bpc.ERR_OVERFLOW
+    got = d_small.download().reshape(ncb, small)
+    assert np.array_equal(got[:, 0], ostream.reshape(ncb, sw)[:, 0])
+    assert (got[:, 1:] == -1).all()
+    c.free()
+    d_cs.free()
+    d_small.free()
+
+
+def test_malformed_msb_flagged(bpc):
+    """A region whose first word is not an MSB (0..27 or 32): byte 255 and malformed status."""
+    sw = 4096
+    regions = np.full((2, sw), -1, np.int32)
+    regions[0, 0] = 5
+    regions[0, 1:4] = [10, 20, 30]
+    regions[1, 0] = 40
+    d = bpc.DeviceBuffer.from_array(regions.ravel())
+    c = bpc.compact_pack(d, 2, sw)
+    assert bpc.status_take() == bpc.ERR_MALFORMED
+    msb, offsets, payload = c.download()
+    assert list(msb) == [5, 255] and list(offsets) == [0, 3, 3] and list(payload) == [10, 20, 30]
+    c.free()
+    d.free()
+
+
+def test_bin_files_from_device(bpc, O, tmp_path):
+    img, p, lut_arr, ostream, oncw = encoded(bpc, O, 2048, 1, 15, 170)
+    ncb, sw = p.n_codeblocks(2048, 2048), p.stride()
+    d_cs = bpc.DeviceBuffer.from_array(ostream)
+    c = bpc.compact_pack(d_cs, ncb, sw)
+    assert bpc.status_take() == 0
+    msb, offsets, payload = c.download()
+    bpc.bin_export(str(tmp_path), msb, offsets, payload, 2048, 2048, p)
+    assert np.array_equal(bpc.bin_import(str(tmp_path), 2048, 2048, p), ostream)
+    c.free()
+    d_cs.free()
