@@ -443,6 +443,51 @@ def bench_container(args, d_cs, d_ncw, ncb, stride, stream):
             "round_trip_exact": exact}
 
 
+def bench_dwt(args, check=True):
+    """Reversible 5/3 transform of a 4096x4096 int32 image, 5 levels, forward and
+    inverse on the device.  HBM-bound; algorithmic bytes: every level reads and
+    writes its band once, 8 * X * Y * (1 + 1/4 + ... + 1/4^4) per direction."""
+    size, levels = 4096, 5
+    rng = np.random.default_rng(args.seed + 500)
+    img = rng.integers(0, 1 << 12, (size, size)).astype(np.int32)
+    stream = bpc.Stream()
+    d_img = bpc.DeviceBuffer.from_array(img)
+    d_coef = bpc.DeviceBuffer(size * size)
+    d_rec = bpc.DeviceBuffer(size * size)
+    bpc.device_sync()
+    ev = [bpc.Event() for _ in range(3)]
+    tf, ti = [], []
+    for k in range(max(1, args.warmup) + max(5, args.steps)):
+        ev[0].record(stream)
+        bpc.dwt53_forward_device(d_img, d_coef, size, size, levels, stream)
+        ev[1].record(stream)
+        bpc.dwt53_inverse_device(d_coef, d_rec, size, size, levels, stream)
+        ev[2].record(stream)
+        if k >= max(1, args.warmup):
+            tf.append(ev[0].elapsed_ms(ev[1]))
+            ti.append(ev[1].elapsed_ms(ev[2]))
+    rec = d_rec.download(stream=stream).reshape(size, size)
+    if not np.array_equal(rec, img):
+        raise RuntimeError("inverse(forward(x)) != x")
+    exact = None
+    if check:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import dwt53
+        exact = bool(np.array_equal(d_coef.download(stream=stream).reshape(size, size), dwt53.forward(img, levels)))
+        if not exact:
+            raise RuntimeError("device transform differs from the oracle")
+    for b in (d_img, d_coef, d_rec):
+        b.free()
+    fwd_ms, inv_ms = float(np.median(tf)), float(np.median(ti))
+    alg = 8.0 * size * size * sum(4.0 ** -l for l in range(levels))
+    return {"workload": "dwt53_4096 (4096x4096 int32, 5 levels)",
+            "forward_ms": round(fwd_ms, 4), "inverse_ms": round(inv_ms, 4),
+            "forward_gbs": round(alg / fwd_ms / 1e6, 1), "inverse_gbs": round(alg / inv_ms / 1e6, 1),
+            "forward_hbm_frac": round(alg / fwd_ms / 1e6 / HBM_PEAK_GBS, 4),
+            "msamples_s_forward_inverse": round(size * size / (fwd_ms + inv_ms) / 1e3, 1),
+            "round_trip_exact": True, "bit_exact_vs_oracle": exact}
+
+
 def bench_train(args):
     """LUT training on the device (symbol statistics of the encoder's modelling, no
     coding) for the plain workload's training image: 4096x4096, 3 passes, 16 LUT
@@ -489,8 +534,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle baseline and parity check")
-    ap.add_argument("--workloads", default="plain,batched,served,large,train,container",
-                    help="comma list of plain, batched, served, large, train, container "
+    ap.add_argument("--workloads", default="plain,batched,served,large,train,container,dwt",
+                    help="comma list of plain, batched, served, large, train, container, dwt "
                          "(plain always runs: it gives `value`)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB 5 times (HBM counter calibration)")
@@ -631,6 +676,8 @@ def main():
         out["large"] = bench_large(args, check=not args.no_cpu)
     if "train" in wl:
         out["lut_trainer"] = bench_train(args)
+    if "dwt" in wl:
+        out["dwt53"] = bench_dwt(args, check=not args.no_cpu)
     print(json.dumps(out), flush=True)
 
 

@@ -136,6 +136,8 @@ def lib():
             "bpc_compact_pack_device": (i32, [P, ctypes.c_int64, i32, P, P, ctypes.c_uint64, P, P]),
             "bpc_compact_unpack_device": (i32, [P, P, P, ctypes.c_uint64, ctypes.c_int64, i32, P, P, P]),
             "bpc_cb_file_order": (i32, [i32, i32, i32, i32, P, i32]),
+            "bpc_dwt53_forward_device": (i32, [P, P, i32, i32, i32, P]),
+            "bpc_dwt53_inverse_device": (i32, [P, P, i32, i32, i32, P]),
             "bpc_bin_export": (i32, [ctypes.c_char_p, P, P, P, i32, i32, PP]),
             "bpc_bin_import": (i32, [ctypes.c_char_p, i32, i32, PP, P]),
             "bpc_encode_device": (i32, [P, P, i32, i32, P, i32, PP, i32, P, P, P]),
@@ -646,3 +648,27 @@ def bin_import(folder, size_x, size_y, p):
     out = np.zeros(stream_words(size_x, size_y, p), dtype=np.int32)
     _check(lib().bpc_bin_import(folder.encode(), size_x, size_y, ctypes.byref(p), _ptr(out)))
     return out
+
+
+# ---- reversible 5/3 wavelet transform ----------------------------------------------
+
+def dwt53_forward_device(d_img, d_coeffs, size_x, size_y, levels, stream=None):
+    _check(lib().bpc_dwt53_forward_device(_dptr(d_img), _dptr(d_coeffs), size_x, size_y, levels, _sp(stream)))
+
+
+def dwt53_inverse_device(d_coeffs, d_img, size_x, size_y, levels, stream=None):
+    _check(lib().bpc_dwt53_inverse_device(_dptr(d_coeffs), _dptr(d_img), size_x, size_y, levels, _sp(stream)))
+
+
+def dwt53(img, levels, inverse=False):
+    """Host convenience: one transform of a (Y, X) int32 image on the device."""
+    img = np.ascontiguousarray(img, dtype=np.int32)
+    Y, X = img.shape
+    a = DeviceBuffer.from_array(img)
+    b = DeviceBuffer(X * Y)
+    try:
+        (dwt53_inverse_device if inverse else dwt53_forward_device)(a, b, X, Y, levels)
+        return b.download().reshape(Y, X)
+    finally:
+        a.free()
+        b.free()

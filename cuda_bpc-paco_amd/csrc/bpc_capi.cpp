@@ -1172,3 +1172,87 @@ int bpc_bin_import(const char* folder, int size_x, int size_y, const bpc_params*
 }
 
 }  // extern "C"
+
+// ---- reversible 5/3 wavelet transform ------------------------------------------
+
+namespace {
+
+int dwt_check(const void* a, const void* b, int size_x, int size_y, int levels) {
+  if (!a || !b) return fail(BPC_ERR_ARG, "null buffer");
+  if (levels < 0 || levels > 16) return fail(BPC_ERR_ARG, "levels must be 0..16");
+  if (size_x <= 0 || size_y <= 0 || size_x % (1 << levels) || size_y % (1 << levels) ||
+      (levels > 0 && ((size_x >> levels) < 1 || (size_y >> levels) < 1)))
+    return fail(BPC_ERR_ARG, "%dx%d is not divisible by 2^%d", size_x, size_y, levels);
+  if ((int64_t)size_x * size_y > (int64_t)1 << 31) return fail(BPC_ERR_ARG, "image too large");
+  const char *pa = (const char*)a, *pb = (const char*)b;
+  const size_t bytes = (size_t)size_x * size_y * sizeof(int32_t);
+  if (pa < pb + bytes && pb < pa + bytes) return fail(BPC_ERR_ARG, "input and output overlap");
+  return BPC_OK;
+}
+
+// Two LL buffers of the library scratch: level l's LL goes to tmp[l % 2]
+// ((X >> (l+1)) x (Y >> (l+1)) samples, so (X/2)(Y/2) and (X/4)(Y/4) suffice).
+int dwt_run(int inverse, const int32_t* in, int32_t* out, int X, int Y, int L, hipStream_t stream) {
+  if (L == 0) {
+    HIP_TRY(hipMemcpyAsync(out, in, (size_t)X * Y * sizeof(int32_t), hipMemcpyDeviceToDevice, stream));
+    return BPC_OK;
+  }
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  const size_t n0 = (size_t)(X / 2) * (Y / 2), n1 = (size_t)(X / 4) * (Y / 4);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  void* base = nullptr;
+  int st = scratch_acquire(dev, (n0 + n1 + 64) * sizeof(int32_t), stream, &base);
+  if (st) return st;
+  int32_t* tmp[2] = {static_cast<int32_t*>(base), static_cast<int32_t*>(base) + ((n0 + 63) & ~(size_t)63)};
+  if (!inverse) {
+    for (int l = 0; l < L; ++l) {
+      bpc::DwtLevel a = {};
+      a.nx = X >> l;
+      a.ny = Y >> l;
+      a.src = l == 0 ? in : tmp[(l - 1) % 2];
+      a.src_pitch = l == 0 ? X : (X >> l);
+      a.ll = l == L - 1 ? out : tmp[l % 2];
+      a.ll_pitch = l == L - 1 ? X : (X >> (l + 1));
+      a.det = out;
+      a.det_pitch = X;
+      hipError_t e = bpc::launch_dwt53_level(0, a, stream);
+      if (e != hipSuccess) return hip_fail(e, "dwt forward launch");
+    }
+  } else {
+    for (int l = L - 1; l >= 0; --l) {
+      bpc::DwtLevel a = {};
+      a.nx = X >> l;
+      a.ny = Y >> l;
+      a.ll = l == L - 1 ? const_cast<int32_t*>(in) : tmp[l % 2];
+      a.ll_pitch = l == L - 1 ? X : (X >> (l + 1));
+      a.det = const_cast<int32_t*>(in);
+      a.det_pitch = X;
+      a.out = l == 0 ? out : tmp[(l - 1) % 2];
+      a.out_pitch = l == 0 ? X : (X >> l);
+      hipError_t e = bpc::launch_dwt53_level(1, a, stream);
+      if (e != hipSuccess) return hip_fail(e, "dwt inverse launch");
+    }
+  }
+  return scratch_release(dev, stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int bpc_dwt53_forward_device(const int32_t* d_img, int32_t* d_coeffs, int size_x, int size_y, int levels,
+                             void* stream) {
+  int st = dwt_check(d_img, d_coeffs, size_x, size_y, levels);
+  if (st) return st;
+  return dwt_run(0, d_img, d_coeffs, size_x, size_y, levels, (hipStream_t)stream);
+}
+
+int bpc_dwt53_inverse_device(const int32_t* d_coeffs, int32_t* d_img, int size_x, int size_y, int levels,
+                             void* stream) {
+  int st = dwt_check(d_coeffs, d_img, size_x, size_y, levels);
+  if (st) return st;
+  return dwt_run(1, d_coeffs, d_img, size_x, size_y, levels, (hipStream_t)stream);
+}
+
+}  // extern "C"

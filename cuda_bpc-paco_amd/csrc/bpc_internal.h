@@ -114,6 +114,21 @@ hipError_t launch_compact_pack(const int32_t* stream, long long ncb, int stride,
 hipError_t launch_compact_unpack(const uint8_t* msb, const unsigned long long* offsets, const uint16_t* payload,
                                  unsigned long long cap, long long ncb, int stride, int32_t* stream, uint32_t* status,
                                  hipStream_t s);
+// One level of the 5/3 transform (bpc_dwt.hip) on an nx x ny band (both even).
+// Forward: src -> ll (LL band) and det (HL / LH / HH around the level's origin).
+// Inverse: ll and det -> out.
+struct DwtLevel {
+  int nx, ny;
+  const int32_t* src;
+  long long src_pitch;
+  int32_t* ll;
+  long long ll_pitch;
+  int32_t* det;
+  long long det_pitch;
+  int32_t* out;
+  long long out_pitch;
+};
+hipError_t launch_dwt53_level(int inverse, const DwtLevel& a, hipStream_t stream);
 // 16-byte aligned device-to-device copy of a multiple of 16 bytes.
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t stream);
 

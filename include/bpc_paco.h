@@ -158,6 +158,19 @@ int bpc_bin_export(const char* folder, const uint8_t* msb, const uint64_t* offse
  * big-endian 16-bit words, -1 past the end of each file. */
 int bpc_bin_import(const char* folder, int size_x, int size_y, const bpc_params* p, int32_t* host_stream);
 
+/* ---- reversible 5/3 wavelet transform (no counterpart in the reference) ----------
+ * The step before encoding and after decoding: the reference codes coefficients
+ * that arrive transformed (its sample image, main_example.cu:71-179).  Integer
+ * lifting with whole-sample symmetric extension (ISO/IEC 15444-1 Annex F): per
+ * level, columns then rows of the current LL band, bands stored in the Mallat
+ * layout Find_Subband (BPC_PaCo.cu:84-111) reads (HL right, LH below, HH diagonal).
+ * Raster int32 size_x*size_y in and out (no overlap); size_x and size_y must be
+ * divisible by 2^levels.  Asynchronous on `stream`; uses the library's scratch. */
+int bpc_dwt53_forward_device(const int32_t* d_img, int32_t* d_coeffs, int size_x, int size_y, int levels,
+                             void* stream);
+int bpc_dwt53_inverse_device(const int32_t* d_coeffs, int32_t* d_img, int size_x, int size_y, int levels,
+                             void* stream);
+
 /* ---- batched codec (many images of uneven size in one persistent launch) --------
  * The codeblocks of all images go through one device work queue: sorted by MSB
  * inside each image and paired there (one wave64 codes two codeblocks of one
