@@ -488,6 +488,59 @@ def bench_dwt(args, check=True):
             "round_trip_exact": True, "bit_exact_vs_oracle": exact}
 
 
+def bench_int16(args):
+    """int16 coefficient path: a 4096x4096 image of at most 15 magnitude bits, 3
+    passes, coded from int16 and from int32 coefficients of equal value (the
+    codestreams must be identical), encode + decode each."""
+    size, cp3, nbp = 4096, 1, 16
+    img, lut_arr, p = make_workload(size, cp3, nbp, mag_bits=15, ll_bits=13, detail_exp=9, seed=args.seed + 700)
+    lut = bpc.Lut.from_array(lut_arr)
+    tiled = synth.to_tiled(img)
+    stream = bpc.Stream()
+    d16 = bpc.DeviceBuffer.from_array(tiled.astype(np.int16))
+    d32 = bpc.DeviceBuffer.from_array(tiled)
+    cs16 = bpc.DeviceBuffer(bpc.stream_words(size, size, p))
+    cs32 = bpc.DeviceBuffer(bpc.stream_words(size, size, p))
+    o16 = bpc.DeviceBuffer(size * size, np.int16)
+    o32 = bpc.DeviceBuffer(size * size)
+    bpc.device_sync()
+    ev = [bpc.Event() for _ in range(3)]
+
+    def run(i16):
+        ev[0].record(stream)
+        if i16:
+            bpc.encode_device_i16(d16, cs16, size, size, lut, p, stream=stream)
+        else:
+            bpc.encode_device(d32, cs32, size, size, lut, p, stream=stream)
+        ev[1].record(stream)
+        if i16:
+            bpc.decode_device_i16(cs16, o16, size, size, lut, p, stream=stream)
+        else:
+            bpc.decode_device(cs32, o32, size, size, lut, p, stream=stream)
+        ev[2].record(stream)
+        return ev[0].elapsed_ms(ev[1]), ev[1].elapsed_ms(ev[2])
+
+    res = {}
+    for i16 in (True, False):
+        t = [run(i16) for _ in range(max(1, args.warmup) + max(5, args.steps // 2))][max(1, args.warmup):]
+        e, d = float(np.median([x for x, _ in t])), float(np.median([y for _, y in t]))
+        res["int16" if i16 else "int32"] = {"encode_ms": round(e, 4), "decode_ms": round(d, 4),
+                                            "msamples_s": round(size * size / (e + d) / 1e3, 1)}
+    if bpc.status_take(None, stream) != 0:
+        raise RuntimeError("int16 leg status: %s" % bpc.lib().bpc_last_error().decode())
+    same = bool(np.array_equal(cs16.download(stream=stream), cs32.download(stream=stream)))
+    exact = bool(np.array_equal(o16.download(stream=stream), tiled.astype(np.int16)))
+    if not (same and exact):
+        raise RuntimeError("int16 path differs from the int32 path")
+    for b in (d16, d32, cs16, cs32, o16, o32):
+        b.free()
+    lut.free()
+    res.update({"workload": "plain_4096_3pass_15bit (int16 vs int32 coefficients, same data)",
+                "coefficient_bytes": {"int16": 2 * size * size, "int32": 4 * size * size},
+                "same_codestream": same, "round_trip_exact": exact})
+    return res
+
+
 def bench_train(args):
     """LUT training on the device (symbol statistics of the encoder's modelling, no
     coding) for the plain workload's training image: 4096x4096, 3 passes, 16 LUT
@@ -534,8 +587,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle baseline and parity check")
-    ap.add_argument("--workloads", default="plain,batched,served,large,train,container,dwt",
-                    help="comma list of plain, batched, served, large, train, container, dwt "
+    ap.add_argument("--workloads", default="plain,batched,served,large,train,container,dwt,int16",
+                    help="comma list of plain, batched, served, large, train, container, dwt, int16 "
                          "(plain always runs: it gives `value`)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB 5 times (HBM counter calibration)")
@@ -676,6 +729,8 @@ def main():
         out["large"] = bench_large(args, check=not args.no_cpu)
     if "train" in wl:
         out["lut_trainer"] = bench_train(args)
+    if "int16" in wl:
+        out["int16"] = bench_int16(args)
     if "dwt" in wl:
         out["dwt53"] = bench_dwt(args, check=not args.no_cpu)
     print(json.dumps(out), flush=True)

@@ -142,6 +142,8 @@ def lib():
             "bpc_bin_import": (i32, [ctypes.c_char_p, i32, i32, PP, P]),
             "bpc_encode_device": (i32, [P, P, i32, i32, P, i32, PP, i32, P, P, P]),
             "bpc_decode_device": (i32, [P, P, i32, i32, P, i32, PP, P, P]),
+            "bpc_encode_device_i16": (i32, [P, P, i32, i32, P, i32, PP, i32, P, P, P]),
+            "bpc_decode_device_i16": (i32, [P, P, i32, i32, P, i32, PP, P, P]),
             "bpc_status_take": (i32, [P, P]),
             "bpc_dev_malloc": (i32, [ctypes.POINTER(P), sz]),
             "bpc_dev_free": (i32, [P]),
@@ -438,6 +440,19 @@ def encode_device(d_coeffs, d_stream, size_x, size_y, lut, p, fill_unused=1, d_n
 def decode_device(d_stream, d_coeffs, size_x, size_y, lut, p, d_status=None, stream=None):
     _check(lib().bpc_decode_device(d_stream.ptr, d_coeffs.ptr, size_x, size_y, lut.ptr, lut.len, ctypes.byref(p),
                                    d_status.ptr if d_status is not None else None, _sp(stream)))
+
+
+def encode_device_i16(d_coeffs, d_stream, size_x, size_y, lut, p, fill_unused=1, d_ncw=None, d_status=None,
+                      stream=None):
+    """encode_device with int16 coefficients (d_coeffs holds size_x*size_y int16)."""
+    _check(lib().bpc_encode_device_i16(_dptr(d_coeffs), _dptr(d_stream), size_x, size_y, lut.ptr, lut.len,
+                                       ctypes.byref(p), fill_unused, _dptr(d_ncw), _dptr(d_status), _sp(stream)))
+
+
+def decode_device_i16(d_stream, d_coeffs, size_x, size_y, lut, p, d_status=None, stream=None):
+    """decode_device into int16 coefficients (saturated; out-of-range values are reported)."""
+    _check(lib().bpc_decode_device_i16(_dptr(d_stream), _dptr(d_coeffs), size_x, size_y, lut.ptr, lut.len,
+                                       ctypes.byref(p), _dptr(d_status), _sp(stream)))
 
 
 def status_take(d_status=None, stream=None):

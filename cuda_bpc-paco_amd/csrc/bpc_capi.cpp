@@ -72,6 +72,7 @@ int status_to_code(uint32_t st) {
     return fail(BPC_ERR_RANGE, "coefficient magnitude >= 2^28, LUT entry outside [0,127] or LUT row outside the array");
   if (st & bpc::ST_OVERFLOW) return fail(BPC_ERR_OVERFLOW, "codestream region capacity exceeded");
   if (st & bpc::ST_WIDE) return fail(BPC_ERR_WIDE, "codeblock MSB beyond the kernel's bitplane registers");
+  if (st & bpc::ST_NARROW) return fail(BPC_ERR_RANGE, "decoded coefficient outside the int16 range (saturated)");
   return BPC_OK;
 }
 
@@ -147,7 +148,8 @@ int scratch_release(int dev, hipStream_t stream) {
 }
 
 int launch(int decode, const int32_t* src, int32_t* dst, int size_x, int size_y, const int32_t* d_lut, int lut_len,
-           const bpc_params* p, int fill, int32_t* d_ncw, uint32_t* d_status, hipStream_t stream, int maxp) {
+           const bpc_params* p, int fill, int32_t* d_ncw, uint32_t* d_status, hipStream_t stream, int maxp,
+           int in16 = 0) {
   bpc::KArgs a;
   a.src = src;
   a.dst = dst;
@@ -165,6 +167,7 @@ int launch(int decode, const int32_t* src, int32_t* dst, int size_x, int size_y,
   a.lut_len = lut_len;
   a.layout = p->layout;
   a.fill = fill;
+  a.in16 = in16;
   if (a.ncb <= 0) return BPC_OK;
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
@@ -425,6 +428,31 @@ int bpc_decode_device(const int32_t* d_stream, int32_t* d_coeffs, int size_x, in
   }
   return launch(1, d_stream, d_coeffs, size_x, size_y, d_lut, lut_len, p, 0, nullptr, d_status,
                 (hipStream_t)stream, bpc::planes_for(p->lut_n_bitplanes));
+}
+
+int bpc_encode_device_i16(const int16_t* d_coeffs, int32_t* d_stream, int size_x, int size_y, const int32_t* d_lut,
+                          int lut_len, const bpc_params* p, int fill_unused, int32_t* d_ncw, uint32_t* d_status,
+                          void* stream) {
+  int st = check_device_call(d_coeffs, d_stream, size_x, size_y, d_lut, lut_len, p);
+  if (st) return st;
+  if (!d_status) {
+    st = default_status(&d_status);
+    if (st) return st;
+  }
+  return launch(0, reinterpret_cast<const int32_t*>(d_coeffs), d_stream, size_x, size_y, d_lut, lut_len, p,
+                fill_unused, d_ncw, d_status, (hipStream_t)stream, bpc::planes_for(p->lut_n_bitplanes), 1);
+}
+
+int bpc_decode_device_i16(const int32_t* d_stream, int16_t* d_coeffs, int size_x, int size_y, const int32_t* d_lut,
+                          int lut_len, const bpc_params* p, uint32_t* d_status, void* stream) {
+  int st = check_device_call(d_stream, d_coeffs, size_x, size_y, d_lut, lut_len, p);
+  if (st) return st;
+  if (!d_status) {
+    st = default_status(&d_status);
+    if (st) return st;
+  }
+  return launch(1, d_stream, reinterpret_cast<int32_t*>(d_coeffs), size_x, size_y, d_lut, lut_len, p, 0, nullptr,
+                d_status, (hipStream_t)stream, bpc::planes_for(p->lut_n_bitplanes), 1);
 }
 
 int bpc_status_take(uint32_t* d_status, void* stream) {

@@ -25,9 +25,13 @@ constexpr int HALO = 2;
 constexpr int W = TILE + 2 * HALO;  // 68 staged rows and columns
 constexpr int LP = W + 1;           // LDS row pitch (odd: column walks hit distinct banks)
 constexpr int THREADS = 256;
+constexpr int NLOAD = (W * W + THREADS - 1) / THREADS;  // staged samples per thread
 
 // Whole-sample symmetric extension of index g into [0, n), n >= 2.
 __device__ __forceinline__ int reflect(int g, int n) {
+  if ((unsigned)g < (unsigned)n) return g;
+  if (g < 0 && -g < n) return -g;
+  if (g >= n && g < 2 * n - 1) return 2 * (n - 1) - g;
   const int p = 2 * (n - 1);
   g %= p;
   if (g < 0) g += p;
@@ -38,10 +42,22 @@ __global__ __launch_bounds__(THREADS) void dwt53_forward_kernel(const DwtLevel a
   __shared__ int32_t t[W][LP];
   const int r0 = blockIdx.y * TILE, c0 = blockIdx.x * TILE;
   const int tid = threadIdx.x;
-  for (int k = tid; k < W * W; k += THREADS) {
-    const int lr = k / W, lc = k - lr * W;
-    const int gr = reflect(r0 - HALO + lr, a.ny), gc = reflect(c0 - HALO + lc, a.nx);
-    t[lr][lc] = a.src[(long long)gr * a.src_pitch + gc];
+  // every load of the thread is issued before the first LDS store, so the tile
+  // costs one HBM round trip rather than one per staged sample
+  int32_t v[NLOAD];
+#pragma unroll
+  for (int q = 0; q < NLOAD; ++q) {
+    const int k = tid + q * THREADS;
+    if (k < W * W) {
+      const int lr = k / W, lc = k - lr * W;
+      const int gr = reflect(r0 - HALO + lr, a.ny), gc = reflect(c0 - HALO + lc, a.nx);
+      v[q] = a.src[(long long)gr * a.src_pitch + gc];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NLOAD; ++q) {
+    const int k = tid + q * THREADS;
+    if (k < W * W) t[k / W][k % W] = v[q];
   }
   __syncthreads();
   // vertical predict: odd staged rows 1..65 (every column)
@@ -86,16 +102,26 @@ __global__ __launch_bounds__(THREADS) void dwt53_inverse_kernel(const DwtLevel a
   const int r0 = blockIdx.y * TILE, c0 = blockIdx.x * TILE;
   const int tid = threadIdx.x;
   const int hy = a.ny >> 1, hx = a.nx >> 1;
-  // interleaved samples y (y[2n] = low band n, y[2n+1] = high band n) from the bands
-  for (int k = tid; k < W * W; k += THREADS) {
-    const int lr = k / W, lc = k - lr * W;
-    const int gr = reflect(r0 - HALO + lr, a.ny), gc = reflect(c0 - HALO + lc, a.nx);
-    int32_t v;
-    if (!(gr & 1) && !(gc & 1))
-      v = a.ll[(long long)(gr >> 1) * a.ll_pitch + (gc >> 1)];
-    else
-      v = a.det[(long long)((gr & 1) ? hy + (gr >> 1) : (gr >> 1)) * a.det_pitch + ((gc & 1) ? hx + (gc >> 1) : (gc >> 1))];
-    t[lr][lc] = v;
+  // interleaved samples y (y[2n] = low band n, y[2n+1] = high band n) from the
+  // bands, all loads in flight before the LDS stores
+  int32_t v[NLOAD];
+#pragma unroll
+  for (int q = 0; q < NLOAD; ++q) {
+    const int k = tid + q * THREADS;
+    if (k < W * W) {
+      const int lr = k / W, lc = k - lr * W;
+      const int gr = reflect(r0 - HALO + lr, a.ny), gc = reflect(c0 - HALO + lc, a.nx);
+      const int32_t* p = (!(gr & 1) && !(gc & 1))
+                             ? a.ll + (long long)(gr >> 1) * a.ll_pitch + (gc >> 1)
+                             : a.det + (long long)((gr & 1) ? hy + (gr >> 1) : (gr >> 1)) * a.det_pitch +
+                                   ((gc & 1) ? hx + (gc >> 1) : (gc >> 1));
+      v[q] = *p;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NLOAD; ++q) {
+    const int k = tid + q * THREADS;
+    if (k < W * W) t[k / W][k % W] = v[q];
   }
   __syncthreads();
   // horizontal inverse update: even columns 2..66, every staged row

@@ -11,6 +11,7 @@ enum : uint32_t {
   ST_OVERFLOW = 2u,   // codeword slots past the codeblock region
   ST_MALFORMED = 4u,  // decoder: MSB word not in [0,27] and not 32
   ST_WIDE = 8u,       // a codeblock needs more bitplanes than the instantiated kernel holds
+  ST_NARROW = 16u,    // decoder: a coefficient outside the int16 output range (saturated)
 };
 
 // One launch over the codeblocks of one image.
@@ -30,6 +31,7 @@ struct KArgs {
   int lut_len;         // ints in lut
   int layout;          // 0: raster (reference), 1: codeblock-contiguous (tile-major)
   int fill;            // encoder: write -1 to unused region words
+  int in16;            // coefficients are int16 (encoder input / decoder output), else int32
   void* scratch;       // per wave of the grid: 2 * MAXP * 64 uint2 (scratch_bytes_per_wave)
 };
 

@@ -363,7 +363,8 @@ struct HalfDesc {
   const int32_t* lut;
   int32_t* ncw;
   int size_x, size_y, nx, stride, nbp, levels, lut_len, layout, fill;
-  int cb;  // codeblock of this half in its image, -1 for an empty half
+  int in16;  // coefficients as int16 instead of int32
+  int cb;    // codeblock of this half in its image, -1 for an empty half
 };
 
 // Codes the two codeblocks of one wave (Kernel_BPC_CODER / Kernel_BPC_DECODER
@@ -397,6 +398,56 @@ struct Scratch {
     out[1] = (u64)lo.y | ((u64)hi.y << 32);
   }
 };
+
+// Coefficients of one lane's stripe (int32 or int16, two's complement) into the
+// wave's bitplane scratch and the sign masks; returns the OR of the magnitudes.
+template <typename T, int MAXP>
+__device__ __forceinline__ u32 read_planes(const int32_t* src32, int64_t cbase, int rs, int rows, bool valid,
+                                           const Scratch<MAXP>& scr, LaneState<MAXP>& s) {
+  const T* src = reinterpret_cast<const T*>(src32);
+  u32 mor = 0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int r0 = 32 * h, r1 = min(rows, 32 * h + 32);
+    u32 plo[MAXP][2];
+#pragma unroll
+    for (int p = 0; p < MAXP; ++p) plo[p][0] = plo[p][1] = 0;
+    u32 glo0 = 0, glo1 = 0;
+#pragma unroll 4
+    for (int r = r0; r < r1; ++r) {
+      int v0 = 0, v1 = 0;
+      if (valid) {
+        const T* q = src + cbase + (int64_t)r * rs;
+        if (sizeof(T) == 2) {
+          const short2 w = *reinterpret_cast<const short2*>(q);
+          v0 = w.x;
+          v1 = w.y;
+        } else {
+          const int2 w = *reinterpret_cast<const int2*>(q);
+          v0 = w.x;
+          v1 = w.y;
+        }
+      }
+      const u32 m0 = v0 < 0 ? 0u - (u32)v0 : (u32)v0;
+      const u32 m1 = v1 < 0 ? 0u - (u32)v1 : (u32)v1;
+      mor |= m0 | m1;
+      const int sh = r - r0;
+      glo0 |= (u32)(v0 < 0) << sh;
+      glo1 |= (u32)(v1 < 0) << sh;
+#pragma unroll
+      for (int p = 0; p < MAXP; ++p) {
+        plo[p][0] |= ((m0 >> p) & 1u) << sh;
+        plo[p][1] |= ((m1 >> p) & 1u) << sh;
+      }
+    }
+    s.G[0] |= (u64)glo0 << (32 * h);
+    s.G[1] |= (u64)glo1 << (32 * h);
+    // plane p, rows of this half, both columns -> the wave's scratch
+#pragma unroll
+    for (int p = 0; p < MAXP; ++p) scr.store(h * MAXP + p, plo[p][0], plo[p][1]);
+  }
+  return mor;
+}
 
 // TRAIN = 1 (with DEC = 0): symbol-count mode.  The encoder's modelling runs
 // unchanged, but every symbol is counted at the LUT entry whose probability would
@@ -452,40 +503,13 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
   bool good = valid;
   if (!DEC) {
     // Read_Coefficients (BPC_PaCo.cu:14-44) straight into bit-sliced planes.
-    u32 mor = 0;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int r0 = 32 * h, r1 = min(rows, 32 * h + 32);
-      u32 plo[MAXP][2];
-#pragma unroll
-      for (int p = 0; p < MAXP; ++p) plo[p][0] = plo[p][1] = 0;
-      u32 glo0 = 0, glo1 = 0;
-#pragma unroll 4
-      for (int r = r0; r < r1; ++r) {
-        int2 v = make_int2(0, 0);
-        if (valid) v = *reinterpret_cast<const int2*>(a.src + cbase + (int64_t)r * rs);
-        const u32 m0 = v.x < 0 ? 0u - (u32)v.x : (u32)v.x;
-        const u32 m1 = v.y < 0 ? 0u - (u32)v.y : (u32)v.y;
-        mor |= m0 | m1;
-        const int sh = r - r0;
-        glo0 |= (u32)(v.x < 0) << sh;
-        glo1 |= (u32)(v.y < 0) << sh;
-#pragma unroll
-        for (int p = 0; p < MAXP; ++p) {
-          plo[p][0] |= ((m0 >> p) & 1u) << sh;
-          plo[p][1] |= ((m1 >> p) & 1u) << sh;
-        }
-      }
-      s.G[0] |= (u64)glo0 << (32 * h);
-      s.G[1] |= (u64)glo1 << (32 * h);
-      // plane p, rows of this half, both columns -> the wave's scratch
-#pragma unroll
-      for (int p = 0; p < MAXP; ++p) scr.store(h * MAXP + p, plo[p][0], plo[p][1]);
-    }
+    const u32 mor = a.in16 ? read_planes<int16_t>(a.src, cbase, rs, rows, valid, scr, s)
+                           : read_planes<int32_t>(a.src, cbase, rs, rows, valid, scr, s);
     // Find_MSB (BPC_PaCo.cu:114-161): OR over the 32 lanes of the half.
+    u32 mall = mor;
 #pragma unroll
-    for (int off = 1; off < 32; off <<= 1) mor |= (u32)__shfl_xor((int)mor, off, 64);
-    msb = mor ? 31 - __clz(mor) : 32;
+    for (int off = 1; off < 32; off <<= 1) mall |= (u32)__shfl_xor((int)mall, off, 64);
+    msb = mall ? 31 - __clz(mall) : 32;
     if (!TRAIN && valid && hl == 0) a.dst[g.region] = msb;  // BPC_PaCo.cu:1155
     if (msb != 32 && msb > 27) {
       if (!TRAIN) status |= ST_RANGE;  // symbol counting skips such a codeblock
@@ -607,7 +631,15 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
           if ((g0 >> sh) & 1u) v0 = -v0;
           if ((g1 >> sh) & 1u) v1 = -v1;
           if (!emit) v0 = v1 = 0;
-          *reinterpret_cast<int2*>(a.dst + cbase + (int64_t)r * rs) = make_int2(v0, v1);
+          if (a.in16) {
+            // int16 output: out-of-range values saturate and are reported
+            const int s0 = max(-32768, min(32767, v0)), s1 = max(-32768, min(32767, v1));
+            if (s0 != v0 || s1 != v1) status |= ST_NARROW;
+            *reinterpret_cast<short2*>(reinterpret_cast<int16_t*>(a.dst) + cbase + (int64_t)r * rs) =
+                make_short2((short)s0, (short)s1);
+          } else {
+            *reinterpret_cast<int2*>(a.dst + cbase + (int64_t)r * rs) = make_int2(v0, v1);
+          }
         }
       }
     }
@@ -635,6 +667,7 @@ __global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
   d.lut_len = a.lut_len;
   d.layout = a.layout;
   d.fill = a.fill;
+  d.in16 = a.in16;
   const int npairs = (a.ncb + 1) / 2;
   for (int pair = wave; pair < npairs; pair += nwaves) {
     const int cb = 2 * pair + ((threadIdx.x & 63) >> 5);
@@ -681,6 +714,7 @@ __global__ __launch_bounds__(256) void bpc_batch_kernel(const BatchArgs a) {
     d.lut_len = im.lut_len;
     d.layout = im.layout;
     d.fill = im.fill;
+    d.in16 = 0;
     d.cb = e == EMPTY ? -1 : (int)(e & 0xFFFFFu);
     if (im.cpass3)
       code_pair<DEC, 1, MAXP>(d, a.rows, &a.ctl[CTL_STATUS], scr);
@@ -818,6 +852,7 @@ __global__ __launch_bounds__(64 * TRAIN_WAVES) void bpc_train_kernel(const Train
   d.lut_len = a.lut_len;
   d.layout = a.layout;
   d.fill = 0;
+  d.in16 = 0;
   const int npairs = (a.ncb + 1) / 2;
   for (int pair = wave; pair < npairs; pair += nwaves) {
     const int cb = 2 * pair + ((threadIdx.x & 63) >> 5);

@@ -118,6 +118,17 @@ int bpc_encode_device(const int32_t* d_coeffs, int32_t* d_stream, int size_x, in
                       void* stream);
 int bpc_decode_device(const int32_t* d_stream, int32_t* d_coeffs, int size_x, int size_y, const int32_t* d_lut,
                       int lut_len, const bpc_params* p, uint32_t* d_status, void* stream);
+/* The same two calls with int16 coefficients (size_x*size_y int16 in p->layout):
+ * half the coefficient bytes for data of at most 15 magnitude bits.  The
+ * codestream is the same as from int32 coefficients of equal value.  The decoder
+ * saturates values outside [-32768, 32767] and reports BPC_ERR_RANGE.  No
+ * counterpart in the reference, whose 16/32-bit switch is unused
+ * (DATATYPE_16BITS_or_32BITS, common.h:110-141: both branches are int). */
+int bpc_encode_device_i16(const int16_t* d_coeffs, int32_t* d_stream, int size_x, int size_y, const int32_t* d_lut,
+                          int lut_len, const bpc_params* p, int fill_unused, int32_t* d_ncw, uint32_t* d_status,
+                          void* stream);
+int bpc_decode_device_i16(const int32_t* d_stream, int16_t* d_coeffs, int size_x, int size_y, const int32_t* d_lut,
+                          int lut_len, const bpc_params* p, uint32_t* d_status, void* stream);
 /* Synchronizes stream, maps the status bits to a BPC_* code and clears them. */
 int bpc_status_take(uint32_t* d_status, void* stream);
 
