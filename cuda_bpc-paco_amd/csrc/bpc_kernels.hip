@@ -256,15 +256,17 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
   }
 }
 
-// One scan step of the refinement pass (MRP_Encoder/MRP_Decoder, BPC_PaCo.cu:593-629).
+// One scan step of the refinement pass (MRP_Encoder/MRP_Decoder, BPC_PaCo.cu:593-629):
+// the coefficients with the refine flag code one bit each.  The reference's
+// "significant without refine flag -> set it" (BPC_PaCo.cu:604) touches only the
+// step's own coefficient, whose flag no later step of the pass reads, so the pass
+// applies it to all coefficients at its end (ref_pass).
 template <int ODD, int DEC, int TRAIN, int MAXP>
 __device__ __forceinline__ void ref_step(LaneState<MAXP>& s, int i, bool act, u64 plane, const PlaneLut& pl,
                                          const Geo& g, const int32_t* src, int32_t* dst) {
   constexpr int c = ODD;
   const u64 bit = 1ull << i;
-  const bool r = (s.R[c] >> i) & 1u;
-  const bool at = act && r;
-  if (act && !r && ((s.S[c] >> i) & 1u)) s.R[c] |= bit;
+  const bool at = act && ((s.R[c] >> i) & 1u);
   if (!TRAIN) reserve<DEC>(s, at, g, src);
   if (at) {
     if (TRAIN) {
@@ -316,14 +318,18 @@ __device__ __forceinline__ void sig_pass(LaneState<MAXP>& s, bool act, const u64
 template <int DEC, int TRAIN, int MAXP>
 __device__ __forceinline__ void ref_pass(LaneState<MAXP>& s, bool act, const u64* plane, const PlaneLut& pl,
                                          const Geo& g, const int32_t* src, int32_t* dst) {
-  const u64 mE = wave_or(act ? s.S[0] : 0ull);
-  const u64 mO = wave_or(act ? s.S[1] : 0ull);
+  const u64 mE = wave_or(act ? s.R[0] : 0ull);  // rows with a refinement symbol in some lane
+  const u64 mO = wave_or(act ? s.R[1] : 0ull);
   u64 rows = mE | mO;
   while (rows) {
     const int i = __builtin_ctzll(rows);
     rows &= rows - 1ull;
     if ((mE >> i) & 1ull) ref_step<0, DEC, TRAIN>(s, i, act, plane[0], pl, g, src, dst);
     if ((mO >> i) & 1ull) ref_step<1, DEC, TRAIN>(s, i, act, plane[1], pl, g, src, dst);
+  }
+  if (act) {  // every significant coefficient refines from the next plane on (BPC_PaCo.cu:604)
+    s.R[0] |= s.S[0];
+    s.R[1] |= s.S[1];
   }
 }
 
