@@ -144,14 +144,21 @@ __device__ __forceinline__ void reserve(LaneState<MAXP>& s, bool at, const Geo& 
   }
 }
 
+// a * b + c with 24-bit a, b (one v_mad_u32_u24; written out because the compiler
+// otherwise widens a multiply by a 0/1 value into a 64-bit multiply-add)
+__device__ __forceinline__ u32 mad24(u32 a, u32 b, u32 c) {
+  u32 r;
+  asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
 // Encoder coder step (BPC_PaCo.cu:281-294).
 template <int MAXP>
 __device__ __forceinline__ void enc_symbol(LaneState<MAXP>& s, u32 sym, u32 prob, const Geo& g, int32_t* dst) {
   const u32 t = __umul24(s.Sz, prob) >> 7;  // Sz < 2^16, prob < 2^7
-  const u32 aux = t + sym;
-  const bool one = sym != 0;
-  s.Sz = one ? s.Sz - aux : t;
-  s.L = one ? s.L + aux : s.L;
+  const u32 aux = t + sym;                   // sym is 0 or 1
+  s.Sz = sym ? s.Sz - aux : t;
+  s.L = mad24(aux, sym, s.L);  // L + aux on a 1 (aux < 2^16: exact in 24 bits)
   if (s.Sz == 0) dst[g.region + s.slot] = (int32_t)s.L;
 }
 
@@ -169,17 +176,43 @@ __device__ __forceinline__ u32 dec_symbol(LaneState<MAXP>& s, u32 prob) {
   return 0u;
 }
 
+// Row i of a 64-row column mask is bit i - 32 H of its 32-bit word H = i / 32.  The
+// scan row is wave-uniform, so each pass runs one loop per word and every bit test
+// and update of a step touches one 32-bit word.
+template <int H>
+__device__ __forceinline__ u32 word(u64 x) {
+  return H ? (u32)(x >> 32) : (u32)x;
+}
+template <int H>
+__device__ __forceinline__ void set_bit(u64& x, u32 b) {
+  x |= H ? ((u64)b << 32) : (u64)b;
+}
+template <int H>
+__device__ __forceinline__ void clear_bit(u64& x, u32 b) {
+  x &= H ? ~((u64)b << 32) : ~(u64)b;
+}
+// 3-row window of a column mask: rows i-1, i, i+1 at bits 0..2 (rows -1 and 64
+// are 0).  W = 0: row 0; W = 1: rows 1..32 (one funnel shift across the words);
+// W = 2: rows 33..63 (one field extract from the upper word).
+template <int W>
+__device__ __forceinline__ u32 window(u64 x, int i) {
+  if (W == 0) return ((u32)x << 1) & 7u;
+  if (W == 1) return __builtin_amdgcn_alignbit((u32)(x >> 32), (u32)x, (u32)(i - 1)) & 7u;
+  return __builtin_amdgcn_ubfe((u32)(x >> 32), (u32)(i - 33), 3u);
+}
+
 // One scan step of the significance pass (CLEAN = 0, SPP_Encoder/SPP_Decoder,
 // BPC_PaCo.cu:412-507) or of the cleanup pass (CLEAN = 1, CP_Encoder/CP_Decoder,
-// BPC_PaCo.cu:510-589) at row i of column ODD.
-template <int ODD, bool FIRST, int DEC, int CP3, int CLEAN, int TRAIN, int MAXP>
+// BPC_PaCo.cu:510-589) at row i (in word H, window form W) of column ODD.
+template <int ODD, int H, int W, int DEC, int CP3, int CLEAN, int TRAIN, int MAXP>
 __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u64 plane, const PlaneLut& pl,
                                          const Geo& g, const int32_t* src, int32_t* dst) {
   constexpr int c = ODD, o = 1 - ODD;
-  const u64 bit = 1ull << i;
-  // 3-row windows (rows i-1, i, i+1 at bits 0..2); rows -1 and past the end are 0.
-  const u32 wo = FIRST ? ((u32)(s.S[o] << 1) & 7u) : ((u32)(s.S[o] >> (i - 1)) & 7u);
-  const u32 wc = FIRST ? ((u32)(s.S[c] << 1) & 7u) : ((u32)(s.S[c] >> (i - 1)) & 7u);
+  const int ib = i - 32 * H;
+  const u32 bit = 1u << ib;
+  // 3-row windows of the other and the own column
+  const u32 wo = window<W>(s.S[o], i);
+  const u32 wc = window<W>(s.S[c], i);
   // the neighbour lane's column (left lane's odd column for an even step, right
   // lane's even column for an odd step; BPC_PaCo.cu:342-388); lanes at the
   // codeblock edge see zeros (Correct_CB_borders, BPC_PaCo.cu:390-410)
@@ -193,14 +226,14 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
     const bool cand = act && !(wc & 2u);
     if (CP3) {
       const bool gate = ctx != 0;  // BPC_PaCo.cu:424
-      if (cand && !gate) s.C[c] |= bit;
+      if (cand && !gate) set_bit<H>(s.C[c], bit);
       at = cand && gate;
     } else {
       at = cand;
     }
   } else {
-    at = act && ((s.C[c] >> i) & 1u);  // BPC_PaCo.cu:515
-    if (at) s.C[c] &= ~bit;            // BPC_PaCo.cu:528
+    at = act && ((word<H>(s.C[c]) >> ib) & 1u);  // BPC_PaCo.cu:515
+    if (at) clear_bit<H>(s.C[c], bit);            // BPC_PaCo.cu:528
   }
   // site 1: significance symbol
   if (!TRAIN) reserve<DEC>(s, at, g, src);
@@ -208,18 +241,18 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
   if (at) {
     const u32 prob = (u32)((CLEAN ? pl.cpsig : pl.sig) >> (7u * ctx)) & 127u;
     if (TRAIN) {
-      sym = (u32)(plane >> i) & 1u;
+      sym = (word<H>(plane) >> ib) & 1u;
       if (pl.tal) tally(g.hist, (CLEAN ? pl.isig + pl.icp : pl.isig) + (int)ctx, sym);
     } else if (!DEC) {
-      sym = (u32)(plane >> i) & 1u;
+      sym = (word<H>(plane) >> ib) & 1u;
       enc_symbol(s, sym, prob, g, dst);
     } else {
       sym = dec_symbol(s, prob);
     }
     if (sym) {
-      s.S[c] |= bit;
-      if (CLEAN) s.R[c] |= bit;  // BPC_PaCo.cu:534
-      if (DEC) s.cur[c] |= bit;  // magnitude bit i of this plane
+      set_bit<H>(s.S[c], bit);
+      if (CLEAN) set_bit<H>(s.R[c], bit);  // BPC_PaCo.cu:534
+      if (DEC) set_bit<H>(s.cur[c], bit);  // magnitude bit i of this plane
     }
   }
   sym = opaque(sym);
@@ -227,11 +260,11 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
   if (ballot(sym != 0)) {
     const bool nsig = sym != 0;
     // neighbour lane's sign bit at row i, moved like its significance window
-    const u32 gs = (u32)(s.G[o] >> i) & 1u;
+    const u32 gs = (word<H>(s.G[o]) >> ib) & 1u;
     const u32 gn = (ODD ? dpp_from_right(gs) : dpp_from_left(gs)) & ((nb >> 1) & 1u);
     // (significant, negative) of the vertical pair U, D (own column rows i-1, i+1)
     // and of the horizontal pair: neighbour lane's column and own other column at row i
-    const u32 gc = FIRST ? ((u32)(s.G[c] << 1) & 5u) : ((u32)(s.G[c] >> (i - 1)) & 5u);
+    const u32 gc = window<W>(s.G[c], i) & 5u;
     const u32 wv = wc & 5u;
     const u32 vidx = wv | ((gc & wv) << 1);
     const u32 so = (wo >> 1) & 1u, go = gs & so;
@@ -242,15 +275,14 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
     if (!TRAIN) reserve<DEC>(s, nsig, g, src);
     if (nsig) {
       const u32 prob = ((CLEAN ? pl.cpsign : pl.sign) >> (8u * (sctx >> 1))) & 127u;
+      const u32 own = (word<H>(s.G[c]) >> ib) & 1u;
       if (TRAIN) {
-        if (pl.tal)
-          tally(g.hist, (CLEAN ? pl.isign + pl.icp : pl.isign) + (int)(sctx >> 1),
-                ((u32)(s.G[c] >> i) & 1u) ^ (sctx & 1u));
+        if (pl.tal) tally(g.hist, (CLEAN ? pl.isign + pl.icp : pl.isign) + (int)(sctx >> 1), own ^ (sctx & 1u));
       } else if (!DEC) {
-        enc_symbol(s, ((u32)(s.G[c] >> i) & 1u) ^ (sctx & 1u), prob, g, dst);
+        enc_symbol(s, own ^ (sctx & 1u), prob, g, dst);
       } else {
         const u32 v = dec_symbol(s, prob) ^ (sctx & 1u);
-        if (v) s.G[c] |= bit;
+        if (v) set_bit<H>(s.G[c], bit);
       }
     }
   }
@@ -261,20 +293,20 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
 // "significant without refine flag -> set it" (BPC_PaCo.cu:604) touches only the
 // step's own coefficient, whose flag no later step of the pass reads, so the pass
 // applies it to all coefficients at its end (ref_pass).
-template <int ODD, int DEC, int TRAIN, int MAXP>
+template <int ODD, int H, int DEC, int TRAIN, int MAXP>
 __device__ __forceinline__ void ref_step(LaneState<MAXP>& s, int i, bool act, u64 plane, const PlaneLut& pl,
                                          const Geo& g, const int32_t* src, int32_t* dst) {
   constexpr int c = ODD;
-  const u64 bit = 1ull << i;
-  const bool at = act && ((s.R[c] >> i) & 1u);
+  const int ib = i - 32 * H;
+  const bool at = act && ((word<H>(s.R[c]) >> ib) & 1u);
   if (!TRAIN) reserve<DEC>(s, at, g, src);
   if (at) {
     if (TRAIN) {
-      if (pl.tal) tally(g.hist, pl.iref, (u32)(plane >> i) & 1u);
+      if (pl.tal) tally(g.hist, pl.iref, (word<H>(plane) >> ib) & 1u);
     } else if (!DEC) {
-      enc_symbol(s, (u32)(plane >> i) & 1u, pl.ref, g, dst);
+      enc_symbol(s, (word<H>(plane) >> ib) & 1u, pl.ref, g, dst);
     } else {
-      if (dec_symbol(s, pl.ref)) s.cur[c] |= bit;
+      if (dec_symbol(s, pl.ref)) set_bit<H>(s.cur[c], 1u << ib);
     }
   }
 }
@@ -294,24 +326,32 @@ __device__ __forceinline__ u64 wave_or(u64 x) {
 // of the wave, change nothing and are skipped.  Row sets are taken at pass start
 // and are supersets of the rows that take part: SPP codes only coefficients that
 // are not significant (significance only grows, one coefficient per step); CP
-// only those with the cleanup flag (never set during CP); MRP only significant ones.
+// only those with the cleanup flag (never set during CP); MRP only those with the
+// refine flag (never set during MRP).  Rows 0-31 and 32-63 run as two loops.
 template <int DEC, int CP3, int CLEAN, int TRAIN, int MAXP>
 __device__ __forceinline__ void sig_pass(LaneState<MAXP>& s, bool act, const u64* plane, const PlaneLut& pl,
                                          const Geo& g, const int32_t* src, int32_t* dst) {
   const u64 rowmask = g.rows >= 64 ? ~0ull : ((1ull << g.rows) - 1ull);
   const u64 mE = wave_or(act ? (CLEAN ? s.C[0] : (~s.S[0] & rowmask)) : 0ull);
   const u64 mO = wave_or(act ? (CLEAN ? s.C[1] : (~s.S[1] & rowmask)) : 0ull);
-  u64 rows = mE | mO;
-  if (rows & 1ull) {
-    if (mE & 1ull) sig_step<0, true, DEC, CP3, CLEAN, TRAIN>(s, 0, act, plane[0], pl, g, src, dst);
-    if (mO & 1ull) sig_step<1, true, DEC, CP3, CLEAN, TRAIN>(s, 0, act, plane[1], pl, g, src, dst);
+  const u32 eL = (u32)mE, oL = (u32)mO, eH = (u32)(mE >> 32), oH = (u32)(mO >> 32);
+  if ((eL | oL) & 1u) {
+    if (eL & 1u) sig_step<0, 0, 0, DEC, CP3, CLEAN, TRAIN>(s, 0, act, plane[0], pl, g, src, dst);
+    if (oL & 1u) sig_step<1, 0, 0, DEC, CP3, CLEAN, TRAIN>(s, 0, act, plane[1], pl, g, src, dst);
   }
-  rows &= ~1ull;
-  while (rows) {
-    const int i = __builtin_ctzll(rows);
-    rows &= rows - 1ull;
-    if ((mE >> i) & 1ull) sig_step<0, false, DEC, CP3, CLEAN, TRAIN>(s, i, act, plane[0], pl, g, src, dst);
-    if ((mO >> i) & 1ull) sig_step<1, false, DEC, CP3, CLEAN, TRAIN>(s, i, act, plane[1], pl, g, src, dst);
+  for (u32 r = (eL | oL) & ~1u; r; r &= r - 1u) {
+    const int i = __builtin_ctz(r);
+    if ((eL >> i) & 1u) sig_step<0, 0, 1, DEC, CP3, CLEAN, TRAIN>(s, i, act, plane[0], pl, g, src, dst);
+    if ((oL >> i) & 1u) sig_step<1, 0, 1, DEC, CP3, CLEAN, TRAIN>(s, i, act, plane[1], pl, g, src, dst);
+  }
+  if ((eH | oH) & 1u) {
+    if (eH & 1u) sig_step<0, 1, 1, DEC, CP3, CLEAN, TRAIN>(s, 32, act, plane[0], pl, g, src, dst);
+    if (oH & 1u) sig_step<1, 1, 1, DEC, CP3, CLEAN, TRAIN>(s, 32, act, plane[1], pl, g, src, dst);
+  }
+  for (u32 r = (eH | oH) & ~1u; r; r &= r - 1u) {
+    const int i = 32 + __builtin_ctz(r);
+    if ((eH >> (i - 32)) & 1u) sig_step<0, 1, 2, DEC, CP3, CLEAN, TRAIN>(s, i, act, plane[0], pl, g, src, dst);
+    if ((oH >> (i - 32)) & 1u) sig_step<1, 1, 2, DEC, CP3, CLEAN, TRAIN>(s, i, act, plane[1], pl, g, src, dst);
   }
 }
 
@@ -320,12 +360,16 @@ __device__ __forceinline__ void ref_pass(LaneState<MAXP>& s, bool act, const u64
                                          const Geo& g, const int32_t* src, int32_t* dst) {
   const u64 mE = wave_or(act ? s.R[0] : 0ull);  // rows with a refinement symbol in some lane
   const u64 mO = wave_or(act ? s.R[1] : 0ull);
-  u64 rows = mE | mO;
-  while (rows) {
-    const int i = __builtin_ctzll(rows);
-    rows &= rows - 1ull;
-    if ((mE >> i) & 1ull) ref_step<0, DEC, TRAIN>(s, i, act, plane[0], pl, g, src, dst);
-    if ((mO >> i) & 1ull) ref_step<1, DEC, TRAIN>(s, i, act, plane[1], pl, g, src, dst);
+  const u32 eL = (u32)mE, oL = (u32)mO, eH = (u32)(mE >> 32), oH = (u32)(mO >> 32);
+  for (u32 r = eL | oL; r; r &= r - 1u) {
+    const int i = __builtin_ctz(r);
+    if ((eL >> i) & 1u) ref_step<0, 0, DEC, TRAIN>(s, i, act, plane[0], pl, g, src, dst);
+    if ((oL >> i) & 1u) ref_step<1, 0, DEC, TRAIN>(s, i, act, plane[1], pl, g, src, dst);
+  }
+  for (u32 r = eH | oH; r; r &= r - 1u) {
+    const int j = __builtin_ctz(r);
+    if ((eH >> j) & 1u) ref_step<0, 1, DEC, TRAIN>(s, 32 + j, act, plane[0], pl, g, src, dst);
+    if ((oH >> j) & 1u) ref_step<1, 1, DEC, TRAIN>(s, 32 + j, act, plane[1], pl, g, src, dst);
   }
   if (act) {  // every significant coefficient refines from the next plane on (BPC_PaCo.cu:604)
     s.R[0] |= s.S[0];
