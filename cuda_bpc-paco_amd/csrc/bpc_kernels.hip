@@ -219,7 +219,7 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
   const u32 nb = (ODD ? dpp_from_right(wo) : dpp_from_left(wo)) & (ODD ? g.edge_odd : g.edge_even);
   // context = significant neighbours; wc includes the coefficient itself, which is
   // not significant wherever a context is used
-  const u32 ctx = __builtin_popcount(wo) + __builtin_popcount(wc) + __builtin_popcount(nb);
+  const u32 ctx = __builtin_popcount(wo | (wc << 3) | (nb << 6));
 
   bool at;
   if (!CLEAN) {
@@ -238,23 +238,24 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
   // site 1: significance symbol
   if (!TRAIN) reserve<DEC>(s, at, g, src);
   u32 sym = 0;
-  if (at) {
-    const u32 prob = (u32)((CLEAN ? pl.cpsig : pl.sig) >> (7u * ctx)) & 127u;
-    if (TRAIN) {
-      sym = (word<H>(plane) >> ib) & 1u;
-      if (pl.tal) tally(g.hist, (CLEAN ? pl.isig + pl.icp : pl.isig) + (int)ctx, sym);
-    } else if (!DEC) {
-      sym = (word<H>(plane) >> ib) & 1u;
-      enc_symbol(s, sym, prob, g, dst);
-    } else {
-      sym = dec_symbol(s, prob);
+  if (!DEC) {
+    const u32 pbit = (word<H>(plane) >> ib) & 1u;  // the symbol, known before coding
+    sym = at ? pbit : 0u;
+    if (at) {
+      if (TRAIN) {
+        if (pl.tal) tally(g.hist, (CLEAN ? pl.isig + pl.icp : pl.isig) + (int)ctx, pbit);
+      } else {
+        enc_symbol(s, pbit, (u32)((CLEAN ? pl.cpsig : pl.sig) >> (7u * ctx)) & 127u, g, dst);
+      }
     }
-    if (sym) {
-      set_bit<H>(s.S[c], bit);
-      if (CLEAN) set_bit<H>(s.R[c], bit);  // BPC_PaCo.cu:534
-      if (DEC) set_bit<H>(s.cur[c], bit);  // magnitude bit i of this plane
-    }
+  } else if (at) {
+    sym = dec_symbol(s, (u32)((CLEAN ? pl.cpsig : pl.sig) >> (7u * ctx)) & 127u);
   }
+  // a 1 makes the coefficient significant (and, in CP, refined: BPC_PaCo.cu:534;
+  // in the decoder it is magnitude bit i of this plane), without a branch
+  set_bit<H>(s.S[c], sym << ib);
+  if (CLEAN) set_bit<H>(s.R[c], sym << ib);
+  if (DEC) set_bit<H>(s.cur[c], sym << ib);
   sym = opaque(sym);
   // site 2: sign symbol of the coefficients that became significant
   if (ballot(sym != 0)) {
