@@ -312,15 +312,29 @@ __device__ __forceinline__ void ref_step(LaneState<MAXP>& s, int i, bool act, u6
   }
 }
 
-// OR of a 64-bit row mask over the wave, wave-uniform.
+// OR of a 32-bit value over the wave, wave-uniform: row_shr 1/2/4/8 leave each
+// 16-lane row's OR in its lane 15, row_bcast:15 folds rows 0 and 2 into rows 1
+// and 3, row_bcast:31 folds row 1 into rows 2 and 3; lane 63 then holds the OR of
+// every lane.  All DPP, no LDS round trips (lanes without a source read 0).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ u32 dpp_or(u32 v) {
+  return v | (u32)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xF, true);
+}
 __device__ __forceinline__ u64 wave_or(u64 x) {
   u32 lo = (u32)x, hi = (u32)(x >> 32);
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    lo |= (u32)__shfl_xor((int)lo, off, 64);
-    hi |= (u32)__shfl_xor((int)hi, off, 64);
-  }
-  return ((u64)__builtin_amdgcn_readfirstlane(hi) << 32) | (u32)__builtin_amdgcn_readfirstlane(lo);
+  lo = dpp_or<0x111, 0xF>(lo);  // row_shr:1
+  hi = dpp_or<0x111, 0xF>(hi);
+  lo = dpp_or<0x112, 0xF>(lo);  // row_shr:2
+  hi = dpp_or<0x112, 0xF>(hi);
+  lo = dpp_or<0x114, 0xF>(lo);  // row_shr:4
+  hi = dpp_or<0x114, 0xF>(hi);
+  lo = dpp_or<0x118, 0xF>(lo);  // row_shr:8
+  hi = dpp_or<0x118, 0xF>(hi);
+  lo = dpp_or<0x142, 0xA>(lo);  // row_bcast:15 into rows 1 and 3
+  hi = dpp_or<0x142, 0xA>(hi);
+  lo = dpp_or<0x143, 0xC>(lo);  // row_bcast:31 into rows 2 and 3
+  hi = dpp_or<0x143, 0xC>(hi);
+  return ((u64)__builtin_amdgcn_readlane(hi, 63) << 32) | (u32)__builtin_amdgcn_readlane(lo, 63);
 }
 
 // Steps whose row holds no coefficient that can take part in the pass, in any lane
