@@ -125,25 +125,41 @@ def bench_batched(args, check=True):
     dec = [bpc.make_image(it["d_cs"], it["d_out"], it["lut"], it["size"], it["size"], it["p"]) for it in items]
     ev = [bpc.Event() for _ in range(3)]
 
+    def use(t):
+        batch.set_tuning(t["codeblocks_per_workgroup"] // 2, 0 if t["waves_per_simd"] == "max" else t["waves_per_simd"])
+        batch.set_vgpr_cap(t["vgpr_cap_waves_per_simd"])
+
+    chosen = {}
+
     def batched_step(graph=True):
         ev[0].record(stream)
+        if chosen:
+            use(chosen["encode"])
         batch.run(bpc.CODE, enc, use_graph=graph, d_status=d_status, stream=stream)
         ev[1].record(stream)
+        if chosen:
+            use(chosen["decode"])
         batch.run(bpc.DECODE, dec, use_graph=graph, d_status=d_status, stream=stream)
         ev[2].record(stream)
         return ev[0].elapsed_ms(ev[1]), ev[1].elapsed_ms(ev[2])
 
-    # autotuner: codeblocks per workgroup (2 per wave) x resident waves per SIMD
+    # autotuner, per direction: codeblocks per workgroup (2 per wave) x resident
+    # waves per SIMD x register budget (compiler's, or capped for 5 waves per SIMD)
     tune = []
     for wpb in (1, 2, 4):
         for wps in (0, 2, 3):
-            batch.set_tuning(wpb, wps)
-            batched_step(graph=False)
-            e, d = batched_step(graph=False)
-            tune.append({"codeblocks_per_workgroup": 2 * wpb, "waves_per_simd": wps or "max",
-                         "encode_ms": round(e, 3), "decode_ms": round(d, 3)})
-    best = min(tune, key=lambda t: t["encode_ms"] + t["decode_ms"])
-    batch.set_tuning(best["codeblocks_per_workgroup"] // 2, 0 if best["waves_per_simd"] == "max" else best["waves_per_simd"])
+            for cap in (0, 5):
+                t = {"codeblocks_per_workgroup": 2 * wpb, "waves_per_simd": wps or "max",
+                     "vgpr_cap_waves_per_simd": cap}
+                use(t)
+                batched_step(graph=False)
+                e, d = batched_step(graph=False)
+                t.update({"encode_ms": round(e, 3), "decode_ms": round(d, 3)})
+                tune.append(t)
+    keys = ("codeblocks_per_workgroup", "waves_per_simd", "vgpr_cap_waves_per_simd")
+    chosen["encode"] = {k: min(tune, key=lambda t: t["encode_ms"])[k] for k in keys}
+    chosen["decode"] = {k: min(tune, key=lambda t: t["decode_ms"])[k] for k in keys}
+    best = dict(chosen)
     rc = bpc.status_take(d_status, stream)
     if rc != 0:
         raise RuntimeError("batched codec status %d: %s" % (rc, bpc.lib().bpc_last_error().decode()))

@@ -170,6 +170,7 @@ def lib():
             "bpc_batch_create": (i32, [ctypes.POINTER(P), i32, ctypes.c_longlong]),
             "bpc_batch_destroy": (i32, [P]),
             "bpc_batch_set_tuning": (i32, [P, i32, i32]),
+            "bpc_batch_set_vgpr_cap": (i32, [P, i32]),
             "bpc_batch_run": (i32, [P, i32, ctypes.POINTER(Image), i32, i32, P, P]),
         }
         for name, (res, args) in sig.items():
@@ -507,6 +508,10 @@ class Batch:
 
     def set_tuning(self, waves_per_block=4, waves_per_simd=0):
         _check(lib().bpc_batch_set_tuning(self.ptr, waves_per_block, waves_per_simd))
+
+    def set_vgpr_cap(self, waves_per_simd=0):
+        """0: the compiler's register budget; 5: registers capped for 5 waves per SIMD."""
+        _check(lib().bpc_batch_set_vgpr_cap(self.ptr, waves_per_simd))
 
     def run(self, direction, images, use_graph=False, d_status=None, stream=None):
         """images: list of Image (or of dicts with the Image fields)."""

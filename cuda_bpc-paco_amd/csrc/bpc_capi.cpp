@@ -789,6 +789,7 @@ struct bpc_batch {
   size_t scratch_bytes = 0;
   int waves_per_block = 4;
   int waves_per_simd = 0;  // 0: as many as fit
+  int vgpr_cap_waves = 0;  // 0: compiler's register budget; 5: capped for 5 waves per SIMD
   int cus = 0;
   std::map<std::string, hipGraphExec_t> graphs;
   std::mutex mu;
@@ -872,7 +873,7 @@ int batch_stage(bpc_batch* b, int direction, const bpc_image* images, int n, Bat
 int batch_grid_blocks(bpc_batch* b, int direction, int maxp, const BatchPlan& plan) {
   if (plan.total_pairs <= 0) return 0;
   const int wpb = b->waves_per_block;
-  int per_cu = bpc::batch_resident_blocks_per_cu(direction == BPC_DECODE, maxp, wpb);
+  int per_cu = bpc::batch_resident_blocks_per_cu(direction == BPC_DECODE, maxp, wpb, b->vgpr_cap_waves);
   if (per_cu <= 0) per_cu = 1;
   if (b->waves_per_simd > 0) per_cu = std::max(1, std::min(per_cu, (4 * b->waves_per_simd) / wpb));
   long long blocks = (long long)per_cu * b->cus;
@@ -916,7 +917,7 @@ int batch_kernels(bpc_batch* b, int direction, int n, const BatchPlan& plan, int
   a.n_pairs = plan.total_pairs;
   a.rows = plan.rows;
   if (blocks > 0) {
-    e = bpc::launch_batch(direction == BPC_DECODE, maxp, a, blocks, b->waves_per_block, stream);
+    e = bpc::launch_batch(direction == BPC_DECODE, maxp, a, blocks, b->waves_per_block, b->vgpr_cap_waves, stream);
     if (e != hipSuccess) return hip_fail(e, "batched kernel launch");
   }
   e = bpc::launch_status_fold(b->d_ctl, d_status, stream);
@@ -976,6 +977,13 @@ int bpc_batch_set_tuning(bpc_batch* b, int waves_per_block, int waves_per_simd) 
   return BPC_OK;
 }
 
+int bpc_batch_set_vgpr_cap(bpc_batch* b, int waves_per_simd) {
+  if (!b || (waves_per_simd != 0 && waves_per_simd != 5)) return fail(BPC_ERR_ARG, "register cap must be 0 or 5");
+  std::lock_guard<std::mutex> lk(b->mu);
+  b->vgpr_cap_waves = waves_per_simd;
+  return BPC_OK;
+}
+
 int bpc_batch_run(bpc_batch* b, int direction, const bpc_image* images, int n_images, int use_graph,
                   uint32_t* d_status, void* stream) {
   if (!b || !images) return fail(BPC_ERR_ARG, "null batch or images");
@@ -1010,7 +1018,8 @@ int bpc_batch_run(bpc_batch* b, int direction, const bpc_image* images, int n_im
     st = batch_kernels(b, direction, n_images, plan, maxp, blocks, d_status, s);
   } else {
     char tun[96];
-    snprintf(tun, sizeof tun, "|%d,%d,%d,%d,%p", b->waves_per_block, b->waves_per_simd, maxp, blocks, (void*)d_status);
+    snprintf(tun, sizeof tun, "|%d,%d,%d,%d,%d,%p", b->waves_per_block, b->waves_per_simd, b->vgpr_cap_waves, maxp, blocks,
+             (void*)d_status);
     const std::string key = plan.signature + tun;
     auto it = b->graphs.find(key);
     if (it == b->graphs.end()) {

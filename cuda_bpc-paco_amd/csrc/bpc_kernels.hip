@@ -751,8 +751,11 @@ constexpr int NBUCKET = 29;  // MSB 27..0, then all-zero / rejected
 __device__ __forceinline__ u32 msb_bucket(int msb) { return (msb >= 0 && msb <= 27) ? (u32)(27 - msb) : (u32)(NBUCKET - 1); }
 
 // Persistent kernel: every wave takes the next pair off the queue until it is drained.
-template <int DEC, int MAXP>
-__global__ __launch_bounds__(256) void bpc_batch_kernel(const BatchArgs a) {
+// OCC = 0 leaves the register budget to the compiler (up to 8 waves per SIMD);
+// OCC = 5 caps it so that 5 waves fit on a SIMD (the launch tuner compares both).
+template <int DEC, int MAXP, int OCC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC ? OCC : 1, OCC ? OCC : 8)))
+void bpc_batch_kernel(const BatchArgs a) {
   const int half = (threadIdx.x & 63) >> 5;
   const size_t wave = (size_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   void* scr = reinterpret_cast<char*>(a.scratch) + wave * (2 * MAXP * 64 * 8);
@@ -1042,43 +1045,56 @@ int codec_resident_waves_per_cu(int decode, int cpass3, int maxp) {
   return cpass3 ? resident_p<0, 1>(maxp) : resident_p<0, 0>(maxp);
 }
 
-template <int DEC, int MAXP>
+template <int DEC, int MAXP, int OCC>
 static hipError_t launch_batch_one(const BatchArgs& a, int grid_blocks, int waves_per_block, hipStream_t stream) {
-  hipLaunchKernelGGL((bpc_batch_kernel<DEC, MAXP>), dim3(grid_blocks), dim3(64 * waves_per_block), 0, stream, a);
+  hipLaunchKernelGGL((bpc_batch_kernel<DEC, MAXP, OCC>), dim3(grid_blocks), dim3(64 * waves_per_block), 0, stream, a);
   return hipGetLastError();
 }
 
-template <int DEC, int MAXP>
+template <int DEC, int MAXP, int OCC>
 static int resident_one(int waves_per_block) {
   int blocks = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, bpc_batch_kernel<DEC, MAXP>, 64 * waves_per_block, 0) !=
-      hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, bpc_batch_kernel<DEC, MAXP, OCC>, 64 * waves_per_block,
+                                                   0) != hipSuccess)
     return 0;
   return blocks;
 }
 
-int batch_resident_blocks_per_cu(int decode, int maxp, int waves_per_block) {
-  switch (maxp * 2 + decode) {
-    case 32: return resident_one<0, 16>(waves_per_block);
-    case 33: return resident_one<1, 16>(waves_per_block);
-    case 40: return resident_one<0, 20>(waves_per_block);
-    case 41: return resident_one<1, 20>(waves_per_block);
-    case 56: return resident_one<0, 28>(waves_per_block);
-    case 57: return resident_one<1, 28>(waves_per_block);
+// Register-capped variants exist for the 16- and 20-plane kernels.
+static int occ_of(int maxp, int vgpr_cap_waves) { return (vgpr_cap_waves == 5 && maxp <= 20) ? 5 : 0; }
+
+int batch_resident_blocks_per_cu(int decode, int maxp, int waves_per_block, int vgpr_cap_waves) {
+  const int occ = occ_of(maxp, vgpr_cap_waves);
+  switch ((maxp * 2 + decode) * 8 + occ) {
+    case 32 * 8: return resident_one<0, 16, 0>(waves_per_block);
+    case 33 * 8: return resident_one<1, 16, 0>(waves_per_block);
+    case 40 * 8: return resident_one<0, 20, 0>(waves_per_block);
+    case 41 * 8: return resident_one<1, 20, 0>(waves_per_block);
+    case 56 * 8: return resident_one<0, 28, 0>(waves_per_block);
+    case 57 * 8: return resident_one<1, 28, 0>(waves_per_block);
+    case 32 * 8 + 5: return resident_one<0, 16, 5>(waves_per_block);
+    case 33 * 8 + 5: return resident_one<1, 16, 5>(waves_per_block);
+    case 40 * 8 + 5: return resident_one<0, 20, 5>(waves_per_block);
+    case 41 * 8 + 5: return resident_one<1, 20, 5>(waves_per_block);
     default: return 0;
   }
 }
 
 hipError_t launch_batch(int decode, int maxp, const BatchArgs& a, int grid_blocks, int waves_per_block,
-                        hipStream_t stream) {
+                        int vgpr_cap_waves, hipStream_t stream) {
   if (waves_per_block < 1 || waves_per_block > 4 || grid_blocks < 1) return hipErrorInvalidValue;
-  switch (maxp * 2 + decode) {
-    case 32: return launch_batch_one<0, 16>(a, grid_blocks, waves_per_block, stream);
-    case 33: return launch_batch_one<1, 16>(a, grid_blocks, waves_per_block, stream);
-    case 40: return launch_batch_one<0, 20>(a, grid_blocks, waves_per_block, stream);
-    case 41: return launch_batch_one<1, 20>(a, grid_blocks, waves_per_block, stream);
-    case 56: return launch_batch_one<0, 28>(a, grid_blocks, waves_per_block, stream);
-    case 57: return launch_batch_one<1, 28>(a, grid_blocks, waves_per_block, stream);
+  const int occ = occ_of(maxp, vgpr_cap_waves);
+  switch ((maxp * 2 + decode) * 8 + occ) {
+    case 32 * 8: return launch_batch_one<0, 16, 0>(a, grid_blocks, waves_per_block, stream);
+    case 33 * 8: return launch_batch_one<1, 16, 0>(a, grid_blocks, waves_per_block, stream);
+    case 40 * 8: return launch_batch_one<0, 20, 0>(a, grid_blocks, waves_per_block, stream);
+    case 41 * 8: return launch_batch_one<1, 20, 0>(a, grid_blocks, waves_per_block, stream);
+    case 56 * 8: return launch_batch_one<0, 28, 0>(a, grid_blocks, waves_per_block, stream);
+    case 57 * 8: return launch_batch_one<1, 28, 0>(a, grid_blocks, waves_per_block, stream);
+    case 32 * 8 + 5: return launch_batch_one<0, 16, 5>(a, grid_blocks, waves_per_block, stream);
+    case 33 * 8 + 5: return launch_batch_one<1, 16, 5>(a, grid_blocks, waves_per_block, stream);
+    case 40 * 8 + 5: return launch_batch_one<0, 20, 5>(a, grid_blocks, waves_per_block, stream);
+    case 41 * 8 + 5: return launch_batch_one<1, 20, 5>(a, grid_blocks, waves_per_block, stream);
     default: return hipErrorInvalidValue;
   }
 }
