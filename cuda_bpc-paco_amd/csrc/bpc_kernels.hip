@@ -752,7 +752,8 @@ __device__ __forceinline__ u32 msb_bucket(int msb) { return (msb >= 0 && msb <= 
 
 // Persistent kernel: every wave takes the next pair off the queue until it is drained.
 // OCC = 0 leaves the register budget to the compiler (up to 8 waves per SIMD);
-// OCC = 5 caps it so that 5 waves fit on a SIMD (the launch tuner compares both).
+// OCC = 5 or 6 caps it so that that many waves fit on a SIMD, at the price of a few
+// spills outside the scan loops (the launch tuner compares them).
 template <int DEC, int MAXP, int OCC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC ? OCC : 1, OCC ? OCC : 8)))
 void bpc_batch_kernel(const BatchArgs a) {
@@ -1061,7 +1062,9 @@ static int resident_one(int waves_per_block) {
 }
 
 // Register-capped variants exist for the 16- and 20-plane kernels.
-static int occ_of(int maxp, int vgpr_cap_waves) { return (vgpr_cap_waves == 5 && maxp <= 20) ? 5 : 0; }
+static int occ_of(int maxp, int vgpr_cap_waves) {
+  return ((vgpr_cap_waves == 5 || vgpr_cap_waves == 6) && maxp <= 20) ? vgpr_cap_waves : 0;
+}
 
 int batch_resident_blocks_per_cu(int decode, int maxp, int waves_per_block, int vgpr_cap_waves) {
   const int occ = occ_of(maxp, vgpr_cap_waves);
@@ -1076,6 +1079,10 @@ int batch_resident_blocks_per_cu(int decode, int maxp, int waves_per_block, int 
     case 33 * 8 + 5: return resident_one<1, 16, 5>(waves_per_block);
     case 40 * 8 + 5: return resident_one<0, 20, 5>(waves_per_block);
     case 41 * 8 + 5: return resident_one<1, 20, 5>(waves_per_block);
+    case 32 * 8 + 6: return resident_one<0, 16, 6>(waves_per_block);
+    case 33 * 8 + 6: return resident_one<1, 16, 6>(waves_per_block);
+    case 40 * 8 + 6: return resident_one<0, 20, 6>(waves_per_block);
+    case 41 * 8 + 6: return resident_one<1, 20, 6>(waves_per_block);
     default: return 0;
   }
 }
@@ -1095,6 +1102,10 @@ hipError_t launch_batch(int decode, int maxp, const BatchArgs& a, int grid_block
     case 33 * 8 + 5: return launch_batch_one<1, 16, 5>(a, grid_blocks, waves_per_block, stream);
     case 40 * 8 + 5: return launch_batch_one<0, 20, 5>(a, grid_blocks, waves_per_block, stream);
     case 41 * 8 + 5: return launch_batch_one<1, 20, 5>(a, grid_blocks, waves_per_block, stream);
+    case 32 * 8 + 6: return launch_batch_one<0, 16, 6>(a, grid_blocks, waves_per_block, stream);
+    case 33 * 8 + 6: return launch_batch_one<1, 16, 6>(a, grid_blocks, waves_per_block, stream);
+    case 40 * 8 + 6: return launch_batch_one<0, 20, 6>(a, grid_blocks, waves_per_block, stream);
+    case 41 * 8 + 6: return launch_batch_one<1, 20, 6>(a, grid_blocks, waves_per_block, stream);
     default: return hipErrorInvalidValue;
   }
 }
