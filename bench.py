@@ -144,11 +144,11 @@ def bench_batched(args, check=True):
         return ev[0].elapsed_ms(ev[1]), ev[1].elapsed_ms(ev[2])
 
     # autotuner, per direction: codeblocks per workgroup (2 per wave) x resident
-    # waves per SIMD x register budget (compiler's, or capped for 5 or 6 waves per SIMD)
+    # waves per SIMD x register budget (compiler's, or capped for 5..8 waves per SIMD)
     tune = []
     for wpb in (1, 2, 4):
         for wps in (0, 2, 3):
-            for cap in (0, 5, 6):
+            for cap in (0, 5, 6, 7, 8):
                 t = {"codeblocks_per_workgroup": 2 * wpb, "waves_per_simd": wps or "max",
                      "vgpr_cap_waves_per_simd": cap}
                 use(t)

@@ -510,7 +510,7 @@ class Batch:
         _check(lib().bpc_batch_set_tuning(self.ptr, waves_per_block, waves_per_simd))
 
     def set_vgpr_cap(self, waves_per_simd=0):
-        """0: the compiler's register budget; 5 or 6: registers capped for that many waves per SIMD."""
+        """0: the compiler's register budget; 5..8: registers capped for that many waves per SIMD."""
         _check(lib().bpc_batch_set_vgpr_cap(self.ptr, waves_per_simd))
 
     def run(self, direction, images, use_graph=False, d_status=None, stream=None):

@@ -789,7 +789,7 @@ struct bpc_batch {
   size_t scratch_bytes = 0;
   int waves_per_block = 4;
   int waves_per_simd = 0;  // 0: as many as fit
-  int vgpr_cap_waves = 0;  // 0: compiler's register budget; 5 or 6: capped for that many waves per SIMD
+  int vgpr_cap_waves = 0;  // 0: compiler's register budget; 5..8: capped for that many waves per SIMD
   int cus = 0;
   std::map<std::string, hipGraphExec_t> graphs;
   std::mutex mu;
@@ -978,8 +978,8 @@ int bpc_batch_set_tuning(bpc_batch* b, int waves_per_block, int waves_per_simd) 
 }
 
 int bpc_batch_set_vgpr_cap(bpc_batch* b, int waves_per_simd) {
-  if (!b || (waves_per_simd != 0 && waves_per_simd != 5 && waves_per_simd != 6))
-    return fail(BPC_ERR_ARG, "register cap must be 0, 5 or 6");
+  if (!b || (waves_per_simd != 0 && (waves_per_simd < 5 || waves_per_simd > 8)))
+    return fail(BPC_ERR_ARG, "register cap must be 0 or 5..8");
   std::lock_guard<std::mutex> lk(b->mu);
   b->vgpr_cap_waves = waves_per_simd;
   return BPC_OK;

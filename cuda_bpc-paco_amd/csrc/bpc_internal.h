@@ -96,7 +96,7 @@ constexpr size_t scratch_bytes_per_wave(int maxp) { return (size_t)2 * maxp * 64
 int codec_resident_waves_per_cu(int decode, int cpass3, int maxp);
 hipError_t launch_codec(int decode, int cpass3, int maxp, const KArgs& a, int grid_waves, hipStream_t stream);
 hipError_t launch_queue_build(const QueueArgs& a, hipStream_t stream);
-// vgpr_cap_waves: 0 (compiler's register budget), 5 or 6 (registers capped for that many waves per SIMD)
+// vgpr_cap_waves: 0 (compiler's register budget) or 5..8 (registers capped for that many waves per SIMD)
 hipError_t launch_batch(int decode, int maxp, const BatchArgs& a, int grid_blocks, int waves_per_block,
                         int vgpr_cap_waves, hipStream_t stream);
 hipError_t launch_status_fold(uint32_t* ctl, uint32_t* status, hipStream_t stream);

@@ -752,8 +752,8 @@ __device__ __forceinline__ u32 msb_bucket(int msb) { return (msb >= 0 && msb <= 
 
 // Persistent kernel: every wave takes the next pair off the queue until it is drained.
 // OCC = 0 leaves the register budget to the compiler (up to 8 waves per SIMD);
-// OCC = 5 or 6 caps it so that that many waves fit on a SIMD, at the price of a few
-// spills outside the scan loops (the launch tuner compares them).
+// OCC = 5..8 caps it so that that many waves fit on a SIMD, at the price of
+// spills (the launch tuner compares them).
 template <int DEC, int MAXP, int OCC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC ? OCC : 1, OCC ? OCC : 8)))
 void bpc_batch_kernel(const BatchArgs a) {
@@ -1063,26 +1063,34 @@ static int resident_one(int waves_per_block) {
 
 // Register-capped variants exist for the 16- and 20-plane kernels.
 static int occ_of(int maxp, int vgpr_cap_waves) {
-  return ((vgpr_cap_waves == 5 || vgpr_cap_waves == 6) && maxp <= 20) ? vgpr_cap_waves : 0;
+  return (vgpr_cap_waves >= 5 && vgpr_cap_waves <= 8 && maxp <= 20) ? vgpr_cap_waves : 0;
 }
 
 int batch_resident_blocks_per_cu(int decode, int maxp, int waves_per_block, int vgpr_cap_waves) {
   const int occ = occ_of(maxp, vgpr_cap_waves);
-  switch ((maxp * 2 + decode) * 8 + occ) {
-    case 32 * 8: return resident_one<0, 16, 0>(waves_per_block);
-    case 33 * 8: return resident_one<1, 16, 0>(waves_per_block);
-    case 40 * 8: return resident_one<0, 20, 0>(waves_per_block);
-    case 41 * 8: return resident_one<1, 20, 0>(waves_per_block);
-    case 56 * 8: return resident_one<0, 28, 0>(waves_per_block);
-    case 57 * 8: return resident_one<1, 28, 0>(waves_per_block);
-    case 32 * 8 + 5: return resident_one<0, 16, 5>(waves_per_block);
-    case 33 * 8 + 5: return resident_one<1, 16, 5>(waves_per_block);
-    case 40 * 8 + 5: return resident_one<0, 20, 5>(waves_per_block);
-    case 41 * 8 + 5: return resident_one<1, 20, 5>(waves_per_block);
-    case 32 * 8 + 6: return resident_one<0, 16, 6>(waves_per_block);
-    case 33 * 8 + 6: return resident_one<1, 16, 6>(waves_per_block);
-    case 40 * 8 + 6: return resident_one<0, 20, 6>(waves_per_block);
-    case 41 * 8 + 6: return resident_one<1, 20, 6>(waves_per_block);
+  switch ((maxp * 2 + decode) * 16 + occ) {
+    case 32 * 16: return resident_one<0, 16, 0>(waves_per_block);
+    case 33 * 16: return resident_one<1, 16, 0>(waves_per_block);
+    case 40 * 16: return resident_one<0, 20, 0>(waves_per_block);
+    case 41 * 16: return resident_one<1, 20, 0>(waves_per_block);
+    case 56 * 16: return resident_one<0, 28, 0>(waves_per_block);
+    case 57 * 16: return resident_one<1, 28, 0>(waves_per_block);
+    case 32 * 16 + 5: return resident_one<0, 16, 5>(waves_per_block);
+    case 33 * 16 + 5: return resident_one<1, 16, 5>(waves_per_block);
+    case 40 * 16 + 5: return resident_one<0, 20, 5>(waves_per_block);
+    case 41 * 16 + 5: return resident_one<1, 20, 5>(waves_per_block);
+    case 32 * 16 + 6: return resident_one<0, 16, 6>(waves_per_block);
+    case 33 * 16 + 6: return resident_one<1, 16, 6>(waves_per_block);
+    case 40 * 16 + 6: return resident_one<0, 20, 6>(waves_per_block);
+    case 41 * 16 + 6: return resident_one<1, 20, 6>(waves_per_block);
+    case 32 * 16 + 7: return resident_one<0, 16, 7>(waves_per_block);
+    case 33 * 16 + 7: return resident_one<1, 16, 7>(waves_per_block);
+    case 40 * 16 + 7: return resident_one<0, 20, 7>(waves_per_block);
+    case 41 * 16 + 7: return resident_one<1, 20, 7>(waves_per_block);
+    case 32 * 16 + 8: return resident_one<0, 16, 8>(waves_per_block);
+    case 33 * 16 + 8: return resident_one<1, 16, 8>(waves_per_block);
+    case 40 * 16 + 8: return resident_one<0, 20, 8>(waves_per_block);
+    case 41 * 16 + 8: return resident_one<1, 20, 8>(waves_per_block);
     default: return 0;
   }
 }
@@ -1091,21 +1099,29 @@ hipError_t launch_batch(int decode, int maxp, const BatchArgs& a, int grid_block
                         int vgpr_cap_waves, hipStream_t stream) {
   if (waves_per_block < 1 || waves_per_block > 4 || grid_blocks < 1) return hipErrorInvalidValue;
   const int occ = occ_of(maxp, vgpr_cap_waves);
-  switch ((maxp * 2 + decode) * 8 + occ) {
-    case 32 * 8: return launch_batch_one<0, 16, 0>(a, grid_blocks, waves_per_block, stream);
-    case 33 * 8: return launch_batch_one<1, 16, 0>(a, grid_blocks, waves_per_block, stream);
-    case 40 * 8: return launch_batch_one<0, 20, 0>(a, grid_blocks, waves_per_block, stream);
-    case 41 * 8: return launch_batch_one<1, 20, 0>(a, grid_blocks, waves_per_block, stream);
-    case 56 * 8: return launch_batch_one<0, 28, 0>(a, grid_blocks, waves_per_block, stream);
-    case 57 * 8: return launch_batch_one<1, 28, 0>(a, grid_blocks, waves_per_block, stream);
-    case 32 * 8 + 5: return launch_batch_one<0, 16, 5>(a, grid_blocks, waves_per_block, stream);
-    case 33 * 8 + 5: return launch_batch_one<1, 16, 5>(a, grid_blocks, waves_per_block, stream);
-    case 40 * 8 + 5: return launch_batch_one<0, 20, 5>(a, grid_blocks, waves_per_block, stream);
-    case 41 * 8 + 5: return launch_batch_one<1, 20, 5>(a, grid_blocks, waves_per_block, stream);
-    case 32 * 8 + 6: return launch_batch_one<0, 16, 6>(a, grid_blocks, waves_per_block, stream);
-    case 33 * 8 + 6: return launch_batch_one<1, 16, 6>(a, grid_blocks, waves_per_block, stream);
-    case 40 * 8 + 6: return launch_batch_one<0, 20, 6>(a, grid_blocks, waves_per_block, stream);
-    case 41 * 8 + 6: return launch_batch_one<1, 20, 6>(a, grid_blocks, waves_per_block, stream);
+  switch ((maxp * 2 + decode) * 16 + occ) {
+    case 32 * 16: return launch_batch_one<0, 16, 0>(a, grid_blocks, waves_per_block, stream);
+    case 33 * 16: return launch_batch_one<1, 16, 0>(a, grid_blocks, waves_per_block, stream);
+    case 40 * 16: return launch_batch_one<0, 20, 0>(a, grid_blocks, waves_per_block, stream);
+    case 41 * 16: return launch_batch_one<1, 20, 0>(a, grid_blocks, waves_per_block, stream);
+    case 56 * 16: return launch_batch_one<0, 28, 0>(a, grid_blocks, waves_per_block, stream);
+    case 57 * 16: return launch_batch_one<1, 28, 0>(a, grid_blocks, waves_per_block, stream);
+    case 32 * 16 + 5: return launch_batch_one<0, 16, 5>(a, grid_blocks, waves_per_block, stream);
+    case 33 * 16 + 5: return launch_batch_one<1, 16, 5>(a, grid_blocks, waves_per_block, stream);
+    case 40 * 16 + 5: return launch_batch_one<0, 20, 5>(a, grid_blocks, waves_per_block, stream);
+    case 41 * 16 + 5: return launch_batch_one<1, 20, 5>(a, grid_blocks, waves_per_block, stream);
+    case 32 * 16 + 6: return launch_batch_one<0, 16, 6>(a, grid_blocks, waves_per_block, stream);
+    case 33 * 16 + 6: return launch_batch_one<1, 16, 6>(a, grid_blocks, waves_per_block, stream);
+    case 40 * 16 + 6: return launch_batch_one<0, 20, 6>(a, grid_blocks, waves_per_block, stream);
+    case 41 * 16 + 6: return launch_batch_one<1, 20, 6>(a, grid_blocks, waves_per_block, stream);
+    case 32 * 16 + 7: return launch_batch_one<0, 16, 7>(a, grid_blocks, waves_per_block, stream);
+    case 33 * 16 + 7: return launch_batch_one<1, 16, 7>(a, grid_blocks, waves_per_block, stream);
+    case 40 * 16 + 7: return launch_batch_one<0, 20, 7>(a, grid_blocks, waves_per_block, stream);
+    case 41 * 16 + 7: return launch_batch_one<1, 20, 7>(a, grid_blocks, waves_per_block, stream);
+    case 32 * 16 + 8: return launch_batch_one<0, 16, 8>(a, grid_blocks, waves_per_block, stream);
+    case 33 * 16 + 8: return launch_batch_one<1, 16, 8>(a, grid_blocks, waves_per_block, stream);
+    case 40 * 16 + 8: return launch_batch_one<0, 20, 8>(a, grid_blocks, waves_per_block, stream);
+    case 41 * 16 + 8: return launch_batch_one<1, 20, 8>(a, grid_blocks, waves_per_block, stream);
     default: return hipErrorInvalidValue;
   }
 }

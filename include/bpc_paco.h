@@ -208,7 +208,7 @@ int bpc_batch_destroy(bpc_batch* batch);
  * or 8 codeblocks per workgroup) and resident waves per SIMD (0 = as many as fit). */
 int bpc_batch_set_tuning(bpc_batch* batch, int waves_per_block, int waves_per_simd);
 /* Register budget of the persistent kernel: 0 = the compiler's (4 waves per SIMD
- * for the coder), 5 or 6 = registers capped so that that many waves fit per SIMD
+ * for the coder), 5..8 = registers capped so that that many waves fit per SIMD
  * (separately compiled variants of the 16- and 20-plane kernels; the 28-plane
  * kernel keeps the compiler's budget).  The results are the same. */
 int bpc_batch_set_vgpr_cap(bpc_batch* batch, int waves_per_simd);

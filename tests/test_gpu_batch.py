@@ -63,7 +63,7 @@ SPECS = [(512, 0, 15, 1, 64), (768, 1, 15, 0, 64), (1024, 1, 16, 1, 64), (512, 1
          (1024, 0, 14, 0, 64), (768, 0, 15, 1, 64), (192, 1, 15, 0, 64)]
 
 
-@pytest.mark.parametrize("wpb,wps,cap", [(4, 0, 0), (1, 2, 0), (2, 3, 0), (4, 0, 5), (1, 0, 5), (2, 0, 6)])
+@pytest.mark.parametrize("wpb,wps,cap", [(4, 0, 0), (1, 2, 0), (2, 3, 0), (4, 0, 5), (1, 0, 5), (2, 0, 6), (1, 0, 7), (4, 0, 8)])
 def test_batch_matches_plain_and_oracle(bpc, O, wpb, wps, cap):
     items = make_set(bpc, SPECS, seed=1000 + wpb + cap)
     batch = bpc.Batch(16, 1 << 16)
