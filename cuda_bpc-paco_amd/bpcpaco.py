@@ -79,7 +79,7 @@ class Image(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     _fields_ = [("max_inflight", ctypes.c_int32), ("max_batch_images", ctypes.c_int32),
                 ("max_batch_codeblocks", ctypes.c_longlong), ("hbm_budget_bytes", ctypes.c_size_t),
-                ("waves_per_block", ctypes.c_int32)]
+                ("waves_per_block", ctypes.c_int32), ("vgpr_cap_waves", ctypes.c_int32)]
 
 
 class Request(ctypes.Structure):
@@ -545,12 +545,13 @@ class Server:
     """Request server: submit encode/decode requests on device buffers, wait for results."""
 
     def __init__(self, max_inflight=2, max_batch_images=64, max_batch_codeblocks=1 << 18, hbm_budget_bytes=32 << 30,
-                 waves_per_block=4):
+                 waves_per_block=4, vgpr_cap_waves=8):
         cfg = ServerConfig()
         lib().bpc_server_default_config(ctypes.byref(cfg))
         cfg.max_inflight, cfg.max_batch_images = max_inflight, max_batch_images
         cfg.max_batch_codeblocks, cfg.hbm_budget_bytes = max_batch_codeblocks, hbm_budget_bytes
         cfg.waves_per_block = waves_per_block
+        cfg.vgpr_cap_waves = vgpr_cap_waves
         p = ctypes.c_void_p()
         _check(lib().bpc_server_create(ctypes.byref(p), ctypes.byref(cfg)))
         self.ptr = p

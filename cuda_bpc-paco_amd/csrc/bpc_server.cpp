@@ -240,6 +240,7 @@ void bpc_server_default_config(bpc_server_config* c) {
   c->max_batch_codeblocks = 1 << 18;
   c->hbm_budget_bytes = (size_t)32 << 30;
   c->waves_per_block = 4;
+  c->vgpr_cap_waves = 8;  // fastest register budget of the batched kernel on MI355X
 }
 
 int bpc_server_create(bpc_server** out, const bpc_server_config* cfg) {
@@ -260,6 +261,7 @@ int bpc_server_create(bpc_server** out, const bpc_server_config* cfg) {
   for (Slot& sl : s->slots) {
     int st = bpc_batch_create(&sl.batch, c.max_batch_images, c.max_batch_codeblocks);
     if (!st) st = bpc_batch_set_tuning(sl.batch, c.waves_per_block, 0);
+    if (!st) st = bpc_batch_set_vgpr_cap(sl.batch, c.vgpr_cap_waves);
     if (!st && hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess) st = BPC_ERR_HIP;
     if (!st && hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess) st = BPC_ERR_HIP;
     if (!st && hipMalloc(&sl.d_status, sizeof(uint32_t)) != hipSuccess) st = BPC_ERR_HIP;

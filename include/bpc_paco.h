@@ -234,6 +234,7 @@ typedef struct bpc_server_config {
   long long max_batch_codeblocks;
   size_t hbm_budget_bytes;      /* output bytes held by admitted, unreleased requests */
   int32_t waves_per_block;      /* tuning of the batched kernel (1, 2 or 4) */
+  int32_t vgpr_cap_waves;       /* its register budget: 0 or 5..8 (bpc_batch_set_vgpr_cap) */
 } bpc_server_config;
 
 typedef struct bpc_request {
