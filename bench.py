@@ -358,6 +358,7 @@ def bench_large(args, check=True):
     stream = bpc.Stream()
     bpc.device_sync()
     batch = bpc.Batch(1, ncb)
+    batch.set_vgpr_cap(args.vgpr_cap)
     enc = [bpc.make_image(d_in, d_cs, lut, size, size, p, d_ncw)]
     dec = [bpc.make_image(d_cs, d_out, lut, size, size, p)]
     ev = [bpc.Event() for _ in range(3)]
@@ -402,6 +403,7 @@ def bench_large(args, check=True):
     wall = time.perf_counter() - t0
     ncw = d_ncw.download(stream=stream)
     out = {"workload": "large_16384 (65536 codeblocks, 20 bitplanes, 3-pass, 8192-word regions, batched path)",
+           "vgpr_cap_waves_per_simd": args.vgpr_cap,
            "msamples_s": round(size * size * steps / wall / 1e6, 1),
            "encode_ms": round(float(np.mean([e for e, _ in times])), 3),
            "decode_ms": round(float(np.mean([d for _, d in times])), 3),
@@ -606,6 +608,9 @@ def main():
     ap.add_argument("--workloads", default="plain,batched,served,large,train,container,dwt,int16",
                     help="comma list of plain, batched, served, large, train, container, dwt, int16 "
                          "(plain always runs: it gives `value`)")
+    ap.add_argument("--vgpr-cap", type=int, default=8,
+                    help="register cap of the batched kernel for the large workload (0 or 5..8; the batched "
+                         "workload's autotuner overrides it when both run)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB 5 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -739,6 +744,7 @@ def main():
     lut.free()
     if "batched" in wl:
         out["batched"] = bench_batched(args, check=not args.no_cpu)
+        args.vgpr_cap = out["batched"]["chosen"]["encode"]["vgpr_cap_waves_per_simd"]
     if "served" in wl:
         out["served"] = bench_served(args)
     if "large" in wl:

@@ -798,6 +798,30 @@ __global__ void queue_reset_kernel(u32* ctl, int words) {
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < words; k += gridDim.x * blockDim.x) ctl[k] = 0;
 }
 
+// Adds 1 to counter[key] for every active lane, with one atomic per distinct key in
+// the wave (the keys of neighbouring threads are mostly equal here: codeblocks and
+// pairs arrive in MSB order), and returns each lane's counter value: the old value
+// plus the lane's rank among the lanes with the same key.
+__device__ __forceinline__ u32 wave_atomic_inc(u32* counter, u32 key) {
+  const int lane = threadIdx.x & 63;
+  u32 result = 0;
+  u64 todo = ballot(true);
+  while (todo) {
+    const int leader = __builtin_ctzll(todo);
+    const u32 lkey = (u32)__builtin_amdgcn_readlane((int)key, leader);
+    const u64 same = ballot(key == lkey) & todo;
+    u32 base = 0;
+    if (lane == leader) base = atomicAdd(counter + lkey, (u32)__builtin_popcountll(same));
+    base = (u32)__builtin_amdgcn_readlane((int)base, leader);
+    if ((same >> lane) & 1ull)
+      result = base + __builtin_amdgcn_mbcnt_hi((u32)(same >> 32), __builtin_amdgcn_mbcnt_lo((u32)same, 0u));
+    todo &= ~same;
+  }
+  return result;
+}
+
+__device__ __forceinline__ u32 mag(int v) { return v < 0 ? 0u - (u32)v : (u32)v; }
+
 // One wave per codeblock: MSB bucket, counted per image.
 __global__ __launch_bounds__(256) void queue_keys_kernel(const QueueArgs a) {
   const long long gcb = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -815,10 +839,18 @@ __global__ __launch_bounds__(256) void queue_keys_kernel(const QueueArgs a) {
     const int tx = cb % im.nx, ty = cb / im.nx;
     const int64_t base = im.layout ? (int64_t)cb * 64 * a.rows : (int64_t)ty * a.rows * im.size_x + tx * 64;
     const int rs = im.layout ? 64 : im.size_x;
+    const int32_t* p0 = im.src + base;
     u32 m = 0;
-    for (int r = 0; r < a.rows; ++r) {
-      const int v = im.src[base + (int64_t)r * rs + lane];
-      m |= v < 0 ? 0u - (u32)v : (u32)v;
+    if ((((uintptr_t)p0 | ((uintptr_t)rs * 4u)) & 15u) == 0) {
+      // 16 bytes per lane: four rows of 64 coefficients per wave load, loads in flight
+      const int col = (lane & 15) * 4;
+#pragma unroll 4
+      for (int r = lane >> 4; r < a.rows; r += 4) {
+        const int4 v = *reinterpret_cast<const int4*>(p0 + (int64_t)r * rs + col);
+        m |= mag(v.x) | mag(v.y) | mag(v.z) | mag(v.w);
+      }
+    } else {
+      for (int r = 0; r < a.rows; ++r) m |= mag(p0[(int64_t)r * rs + lane]);
     }
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) m |= (u32)__shfl_xor((int)m, off, 64);
@@ -856,7 +888,7 @@ __global__ __launch_bounds__(256) void queue_scatter_kernel(const QueueArgs a) {
     const int mid = (lo + hi + 1) >> 1;
     if (a.cb_offset[mid] <= gcb) lo = mid; else hi = mid - 1;
   }
-  const u32 pos = atomicAdd(&a.ctl[CTL_IMG + lo * 32 + a.keys[gcb]], 1u);
+  const u32 pos = wave_atomic_inc(a.ctl + CTL_IMG, (u32)lo * 32u + a.keys[gcb]);
   a.sorted[pos] = ((u32)lo << 20) | (u32)(gcb - a.cb_offset[lo]);
 }
 
@@ -869,7 +901,7 @@ __global__ __launch_bounds__(256) void queue_pair_keys_kernel(const QueueArgs a)
   const u32 b = a.keys[a.cb_offset[img] + (e & 0xFFFFFu)];
   const u32 k = (a.images[img].cpass3 ? NBUCKET : 0) + b;
   a.pair_keys[p] = k;
-  atomicAdd(&a.ctl[CTL_PAIRHIST + k], 1u);
+  (void)wave_atomic_inc(a.ctl + CTL_PAIRHIST, k);
 }
 
 __global__ void queue_pair_scan_kernel(const QueueArgs a) {
@@ -885,7 +917,7 @@ __global__ void queue_pair_scan_kernel(const QueueArgs a) {
 __global__ __launch_bounds__(256) void queue_pair_scatter_kernel(const QueueArgs a) {
   const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= a.total_pairs) return;
-  const u32 pos = atomicAdd(&a.ctl[CTL_PAIRHIST + a.pair_keys[p]], 1u);
+  const u32 pos = wave_atomic_inc(a.ctl + CTL_PAIRHIST, a.pair_keys[p]);
   a.pair_queue[pos] = (u32)p;
 }
 
