@@ -320,6 +320,15 @@ template <int CTRL, int ROW_MASK>
 __device__ __forceinline__ u32 dpp_or(u32 v) {
   return v | (u32)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xF, true);
 }
+__device__ __forceinline__ u32 wave_or32(u32 v) {
+  v = dpp_or<0x111, 0xF>(v);
+  v = dpp_or<0x112, 0xF>(v);
+  v = dpp_or<0x114, 0xF>(v);
+  v = dpp_or<0x118, 0xF>(v);
+  v = dpp_or<0x142, 0xA>(v);
+  v = dpp_or<0x143, 0xC>(v);
+  return (u32)__builtin_amdgcn_readlane((int)v, 63);
+}
 __device__ __forceinline__ u64 wave_or(u64 x) {
   u32 lo = (u32)x, hi = (u32)(x >> 32);
   lo = dpp_or<0x111, 0xF>(lo);  // row_shr:1
@@ -822,7 +831,7 @@ __device__ __forceinline__ u32 wave_atomic_inc(u32* counter, u32 key) {
 
 __device__ __forceinline__ u32 mag(int v) { return v < 0 ? 0u - (u32)v : (u32)v; }
 
-// One wave per codeblock: MSB bucket, counted per image.
+// Encoder keys: one wave per codeblock ORs its magnitudes; MSB bucket, counted per image.
 __global__ __launch_bounds__(256) void queue_keys_kernel(const QueueArgs a) {
   const long long gcb = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
@@ -835,7 +844,7 @@ __global__ __launch_bounds__(256) void queue_keys_kernel(const QueueArgs a) {
   const BatchImage& im = a.images[lo];
   const int cb = (int)(gcb - a.cb_offset[lo]);
   int msb;
-  if (!a.decode) {
+  {
     const int tx = cb % im.nx, ty = cb / im.nx;
     const int64_t base = im.layout ? (int64_t)cb * 64 * a.rows : (int64_t)ty * a.rows * im.size_x + tx * 64;
     const int rs = im.layout ? 64 : im.size_x;
@@ -852,17 +861,29 @@ __global__ __launch_bounds__(256) void queue_keys_kernel(const QueueArgs a) {
     } else {
       for (int r = 0; r < a.rows; ++r) m |= mag(p0[(int64_t)r * rs + lane]);
     }
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) m |= (u32)__shfl_xor((int)m, off, 64);
+    m = wave_or32(m);
     msb = m ? 31 - __clz(m) : 32;
-  } else {
-    msb = im.src[(int64_t)cb * im.stride];
   }
   if (lane == 0) {
     const u32 b = msb_bucket(msb);
     a.keys[gcb] = b;
     atomicAdd(&a.ctl[CTL_IMG + lo * 32 + b], 1u);
   }
+}
+
+// Decoder keys: one thread per codeblock reads the MSB word of its region.
+__global__ __launch_bounds__(256) void queue_keys_decode_kernel(const QueueArgs a) {
+  const long long gcb = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gcb >= a.total_cbs) return;
+  int lo = 0, hi = a.n_images - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a.cb_offset[mid] <= gcb) lo = mid; else hi = mid - 1;
+  }
+  const BatchImage& im = a.images[lo];
+  const u32 b = msb_bucket(im.src[(gcb - a.cb_offset[lo]) * im.stride]);
+  a.keys[gcb] = b;
+  (void)wave_atomic_inc(a.ctl + CTL_IMG, (u32)lo * 32u + b);
 }
 
 // One thread per image: bucket cursors inside the image's (even-padded) range.
@@ -1162,7 +1183,11 @@ hipError_t launch_queue_build(const QueueArgs& a, hipStream_t stream) {
   const int words = CTL_IMG + 32 * a.n_images;
   hipLaunchKernelGGL(queue_reset_kernel, dim3((words + 255) / 256), dim3(256), 0, stream, a.ctl, words);
   if (a.total_cbs > 0) {
-    hipLaunchKernelGGL(queue_keys_kernel, dim3((unsigned)((a.total_cbs + 3) / 4)), dim3(256), 0, stream, a);
+    if (a.decode)
+      hipLaunchKernelGGL(queue_keys_decode_kernel, dim3((unsigned)((a.total_cbs + 255) / 256)), dim3(256), 0, stream,
+                         a);
+    else
+      hipLaunchKernelGGL(queue_keys_kernel, dim3((unsigned)((a.total_cbs + 3) / 4)), dim3(256), 0, stream, a);
     hipLaunchKernelGGL(queue_image_scan_kernel, dim3((a.n_images + 63) / 64), dim3(64), 0, stream, a);
     hipLaunchKernelGGL(queue_scatter_kernel, dim3((unsigned)((a.total_cbs + 255) / 256)), dim3(256), 0, stream, a);
     hipLaunchKernelGGL(queue_pair_keys_kernel, dim3((unsigned)((a.total_pairs + 255) / 256)), dim3(256), 0, stream, a);
