@@ -52,6 +52,9 @@ enum : int {
   CTL_IMG = 128,     // per image: 29 MSB-bucket counts, then cursors (32 words each)
 };
 
+constexpr uint32_t QUEUE_EMPTY = 0xFFFFFFFFu;  // the missing partner of an odd codeblock count
+constexpr int NBUCKET = 29;                      // MSB buckets: 27..0, then all-zero / rejected
+
 struct QueueArgs {
   const BatchImage* images;
   const long long* cb_offset;    // first global codeblock of each image (n_images + 1 entries)

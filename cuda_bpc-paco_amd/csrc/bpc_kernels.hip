@@ -320,15 +320,6 @@ template <int CTRL, int ROW_MASK>
 __device__ __forceinline__ u32 dpp_or(u32 v) {
   return v | (u32)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xF, true);
 }
-__device__ __forceinline__ u32 wave_or32(u32 v) {
-  v = dpp_or<0x111, 0xF>(v);
-  v = dpp_or<0x112, 0xF>(v);
-  v = dpp_or<0x114, 0xF>(v);
-  v = dpp_or<0x118, 0xF>(v);
-  v = dpp_or<0x142, 0xA>(v);
-  v = dpp_or<0x143, 0xC>(v);
-  return (u32)__builtin_amdgcn_readlane((int)v, 63);
-}
 __device__ __forceinline__ u64 wave_or(u64 x) {
   u32 lo = (u32)x, hi = (u32)(x >> 32);
   lo = dpp_or<0x111, 0xF>(lo);  // row_shr:1
@@ -754,11 +745,6 @@ __global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
 // Codeblocks are sorted by MSB inside each image and paired there (both halves of
 // a wave always code the same image, so its descriptor is wave-uniform); pairs
 // are then ordered by (pass mode, MSB) with the costliest first.
-constexpr u32 EMPTY = 0xFFFFFFFFu;
-constexpr int NBUCKET = 29;  // MSB 27..0, then all-zero / rejected
-
-__device__ __forceinline__ u32 msb_bucket(int msb) { return (msb >= 0 && msb <= 27) ? (u32)(27 - msb) : (u32)(NBUCKET - 1); }
-
 // Persistent kernel: every wave takes the next pair off the queue until it is drained.
 // OCC = 0 leaves the register budget to the compiler (up to 8 waves per SIMD);
 // OCC = 5..8 caps it so that that many waves fit on a SIMD, at the price of
@@ -793,158 +779,12 @@ void bpc_batch_kernel(const BatchArgs a) {
     d.layout = im.layout;
     d.fill = im.fill;
     d.in16 = 0;
-    d.cb = e == EMPTY ? -1 : (int)(e & 0xFFFFFu);
+    d.cb = e == QUEUE_EMPTY ? -1 : (int)(e & 0xFFFFFu);
     if (im.cpass3)
       code_pair<DEC, 1, MAXP>(d, a.rows, &a.ctl[CTL_STATUS], scr);
     else
       code_pair<DEC, 0, MAXP>(d, a.rows, &a.ctl[CTL_STATUS], scr);
   }
-}
-
-// Clears the queue counters (kernels rather than memsets, so captured graphs
-// hold no memset nodes).
-__global__ void queue_reset_kernel(u32* ctl, int words) {
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < words; k += gridDim.x * blockDim.x) ctl[k] = 0;
-}
-
-// Adds 1 to counter[key] for every active lane, with one atomic per distinct key in
-// the wave (the keys of neighbouring threads are mostly equal here: codeblocks and
-// pairs arrive in MSB order), and returns each lane's counter value: the old value
-// plus the lane's rank among the lanes with the same key.
-__device__ __forceinline__ u32 wave_atomic_inc(u32* counter, u32 key) {
-  const int lane = threadIdx.x & 63;
-  u32 result = 0;
-  u64 todo = ballot(true);
-  while (todo) {
-    const int leader = __builtin_ctzll(todo);
-    const u32 lkey = (u32)__builtin_amdgcn_readlane((int)key, leader);
-    const u64 same = ballot(key == lkey) & todo;
-    u32 base = 0;
-    if (lane == leader) base = atomicAdd(counter + lkey, (u32)__builtin_popcountll(same));
-    base = (u32)__builtin_amdgcn_readlane((int)base, leader);
-    if ((same >> lane) & 1ull)
-      result = base + __builtin_amdgcn_mbcnt_hi((u32)(same >> 32), __builtin_amdgcn_mbcnt_lo((u32)same, 0u));
-    todo &= ~same;
-  }
-  return result;
-}
-
-__device__ __forceinline__ u32 mag(int v) { return v < 0 ? 0u - (u32)v : (u32)v; }
-
-// Encoder keys: one wave per codeblock ORs its magnitudes; MSB bucket, counted per image.
-__global__ __launch_bounds__(256) void queue_keys_kernel(const QueueArgs a) {
-  const long long gcb = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (gcb >= a.total_cbs) return;  // wave-uniform
-  int lo = 0, hi = a.n_images - 1;  // image owning gcb: last k with cb_offset[k] <= gcb
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (a.cb_offset[mid] <= gcb) lo = mid; else hi = mid - 1;
-  }
-  const BatchImage& im = a.images[lo];
-  const int cb = (int)(gcb - a.cb_offset[lo]);
-  int msb;
-  {
-    const int tx = cb % im.nx, ty = cb / im.nx;
-    const int64_t base = im.layout ? (int64_t)cb * 64 * a.rows : (int64_t)ty * a.rows * im.size_x + tx * 64;
-    const int rs = im.layout ? 64 : im.size_x;
-    const int32_t* p0 = im.src + base;
-    u32 m = 0;
-    if ((((uintptr_t)p0 | ((uintptr_t)rs * 4u)) & 15u) == 0) {
-      // 16 bytes per lane: four rows of 64 coefficients per wave load, loads in flight
-      const int col = (lane & 15) * 4;
-#pragma unroll 4
-      for (int r = lane >> 4; r < a.rows; r += 4) {
-        const int4 v = *reinterpret_cast<const int4*>(p0 + (int64_t)r * rs + col);
-        m |= mag(v.x) | mag(v.y) | mag(v.z) | mag(v.w);
-      }
-    } else {
-      for (int r = 0; r < a.rows; ++r) m |= mag(p0[(int64_t)r * rs + lane]);
-    }
-    m = wave_or32(m);
-    msb = m ? 31 - __clz(m) : 32;
-  }
-  if (lane == 0) {
-    const u32 b = msb_bucket(msb);
-    a.keys[gcb] = b;
-    atomicAdd(&a.ctl[CTL_IMG + lo * 32 + b], 1u);
-  }
-}
-
-// Decoder keys: one thread per codeblock reads the MSB word of its region.
-__global__ __launch_bounds__(256) void queue_keys_decode_kernel(const QueueArgs a) {
-  const long long gcb = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gcb >= a.total_cbs) return;
-  int lo = 0, hi = a.n_images - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (a.cb_offset[mid] <= gcb) lo = mid; else hi = mid - 1;
-  }
-  const BatchImage& im = a.images[lo];
-  const u32 b = msb_bucket(im.src[(gcb - a.cb_offset[lo]) * im.stride]);
-  a.keys[gcb] = b;
-  (void)wave_atomic_inc(a.ctl + CTL_IMG, (u32)lo * 32u + b);
-}
-
-// One thread per image: bucket cursors inside the image's (even-padded) range.
-__global__ void queue_image_scan_kernel(const QueueArgs a) {
-  const int img = blockIdx.x * blockDim.x + threadIdx.x;
-  if (img >= a.n_images) return;
-  u32 pos = (u32)a.sorted_base[img];
-  for (int b = 0; b < NBUCKET; ++b) {
-    u32* c = &a.ctl[CTL_IMG + img * 32 + b];
-    const u32 n = *c;
-    *c = pos;
-    pos += n;
-  }
-  const long long ncb = a.cb_offset[img + 1] - a.cb_offset[img];
-  if (ncb & 1) a.sorted[pos] = EMPTY;
-}
-
-__global__ __launch_bounds__(256) void queue_scatter_kernel(const QueueArgs a) {
-  const long long gcb = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gcb >= a.total_cbs) return;
-  int lo = 0, hi = a.n_images - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (a.cb_offset[mid] <= gcb) lo = mid; else hi = mid - 1;
-  }
-  const u32 pos = wave_atomic_inc(a.ctl + CTL_IMG, (u32)lo * 32u + a.keys[gcb]);
-  a.sorted[pos] = ((u32)lo << 20) | (u32)(gcb - a.cb_offset[lo]);
-}
-
-// One thread per pair: key (pass mode, bucket of its costlier codeblock).
-__global__ __launch_bounds__(256) void queue_pair_keys_kernel(const QueueArgs a) {
-  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.total_pairs) return;
-  const u32 e = a.sorted[2 * p];
-  const int img = (int)(e >> 20);
-  const u32 b = a.keys[a.cb_offset[img] + (e & 0xFFFFFu)];
-  const u32 k = (a.images[img].cpass3 ? NBUCKET : 0) + b;
-  a.pair_keys[p] = k;
-  (void)wave_atomic_inc(a.ctl + CTL_PAIRHIST, k);
-}
-
-__global__ void queue_pair_scan_kernel(const QueueArgs a) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  u32 pos = 0;
-  for (int k = 0; k < 2 * NBUCKET; ++k) {
-    const u32 n = a.ctl[CTL_PAIRHIST + k];
-    a.ctl[CTL_PAIRHIST + k] = pos;
-    pos += n;
-  }
-}
-
-__global__ __launch_bounds__(256) void queue_pair_scatter_kernel(const QueueArgs a) {
-  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.total_pairs) return;
-  const u32 pos = wave_atomic_inc(a.ctl + CTL_PAIRHIST, a.pair_keys[p]);
-  a.pair_queue[pos] = (u32)p;
-}
-
-// Folds the batch's status into the caller's status word.
-__global__ void status_fold_kernel(u32* ctl, uint32_t* status) {
-  if (threadIdx.x == 0 && ctl[CTL_STATUS]) atomicOr(status, ctl[CTL_STATUS]);
 }
 
 // ---- LUT trainer ------------------------------------------------------------
@@ -1177,30 +1017,6 @@ hipError_t launch_batch(int decode, int maxp, const BatchArgs& a, int grid_block
     case 41 * 16 + 8: return launch_batch_one<1, 20, 8>(a, grid_blocks, waves_per_block, stream);
     default: return hipErrorInvalidValue;
   }
-}
-
-hipError_t launch_queue_build(const QueueArgs& a, hipStream_t stream) {
-  const int words = CTL_IMG + 32 * a.n_images;
-  hipLaunchKernelGGL(queue_reset_kernel, dim3((words + 255) / 256), dim3(256), 0, stream, a.ctl, words);
-  if (a.total_cbs > 0) {
-    if (a.decode)
-      hipLaunchKernelGGL(queue_keys_decode_kernel, dim3((unsigned)((a.total_cbs + 255) / 256)), dim3(256), 0, stream,
-                         a);
-    else
-      hipLaunchKernelGGL(queue_keys_kernel, dim3((unsigned)((a.total_cbs + 3) / 4)), dim3(256), 0, stream, a);
-    hipLaunchKernelGGL(queue_image_scan_kernel, dim3((a.n_images + 63) / 64), dim3(64), 0, stream, a);
-    hipLaunchKernelGGL(queue_scatter_kernel, dim3((unsigned)((a.total_cbs + 255) / 256)), dim3(256), 0, stream, a);
-    hipLaunchKernelGGL(queue_pair_keys_kernel, dim3((unsigned)((a.total_pairs + 255) / 256)), dim3(256), 0, stream, a);
-    hipLaunchKernelGGL(queue_pair_scan_kernel, dim3(1), dim3(64), 0, stream, a);
-    hipLaunchKernelGGL(queue_pair_scatter_kernel, dim3((unsigned)((a.total_pairs + 255) / 256)), dim3(256), 0, stream,
-                       a);
-  }
-  return hipGetLastError();
-}
-
-hipError_t launch_status_fold(uint32_t* ctl, uint32_t* status, hipStream_t stream) {
-  hipLaunchKernelGGL(status_fold_kernel, dim3(1), dim3(64), 0, stream, ctl, status);
-  return hipGetLastError();
 }
 
 hipError_t launch_codec(int decode, int cpass3, int maxp, const KArgs& a, int grid_waves, hipStream_t stream) {
