@@ -52,8 +52,9 @@ int check_params(const bpc_params* p) {
   if (p->dwt_levels < 0 || p->dwt_levels > 16) return fail(BPC_ERR_ARG, "dwt_levels out of range");
   if (p->cblock_length < 2 || p->cblock_length > 64) return fail(BPC_ERR_ARG, "cblock_length must be 2..64");
   if (p->lut_n_bitplanes < 1 || p->lut_n_bitplanes > 28) return fail(BPC_ERR_ARG, "lut_n_bitplanes must be 1..28");
-  if (p->cb_stride_words < 0 || (p->cb_stride_words > 0 && p->cb_stride_words < 2))
-    return fail(BPC_ERR_ARG, "cb_stride_words must be 0 or >= 2");
+  if (p->cb_stride_words < 0 || (p->cb_stride_words > 0 && p->cb_stride_words < 2) ||
+      p->cb_stride_words > (1 << 26))
+    return fail(BPC_ERR_ARG, "cb_stride_words must be 0 or 2..2^26");
   if (p->layout != BPC_LAYOUT_RASTER && p->layout != BPC_LAYOUT_TILED) return fail(BPC_ERR_ARG, "bad layout");
   return BPC_OK;
 }
@@ -173,8 +174,12 @@ int launch(int decode, const int32_t* src, int32_t* dst, int size_x, int size_y,
   HIP_TRY(hipGetDevice(&dev));
   int per_cu = bpc::codec_resident_waves_per_cu(decode, p->cpass3, maxp);
   if (per_cu <= 0) per_cu = 1;
-  const long long pairs = (a.ncb + 1) / 2;
-  const int grid = (int)std::min<long long>(pairs, (long long)per_cu * device_cus(dev));
+  // An image with no more codeblocks than the device has SIMDs cannot fill it with
+  // pairs: one codeblock per wave then runs each codeblock's chain on its own SIMD
+  // instead of two in lock step (the launch time is the longest chain).
+  a.single = (long long)a.ncb <= 4LL * device_cus(dev) ? 1 : 0;
+  const long long units = a.single ? a.ncb : (a.ncb + 1) / 2;
+  const int grid = (int)std::min<long long>(units, (long long)per_cu * device_cus(dev));
   const size_t need = (size_t)grid * bpc::scratch_bytes_per_wave(maxp);
   std::lock_guard<std::mutex> lk(g_scratch_mu);
   int st = scratch_acquire(dev, need, stream, &a.scratch);

@@ -31,6 +31,7 @@ struct KArgs {
   int lut_len;         // ints in lut
   int layout;          // 0: raster (reference), 1: codeblock-contiguous (tile-major)
   int fill;            // encoder: write -1 to unused region words
+  int single;          // one codeblock per wave (lanes 32-63 idle) instead of a pair
   int in16;            // coefficients are int16 (encoder input / decoder output), else int32
   void* scratch;       // per wave of the grid: 2 * MAXP * 64 uint2 (scratch_bytes_per_wave)
 };

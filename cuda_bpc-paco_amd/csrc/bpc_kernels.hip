@@ -108,7 +108,13 @@ struct Geo {
   u32 edge_even;   // 7, or 0 for the first lane of a half (no left neighbour in the codeblock)
   u32 edge_odd;    // 7, or 0 for the last lane of a half (no right neighbour)
   unsigned long long* hist;  // symbol-count mode: per LUT entry, (symbols << 32) + zeros, in LDS
+  // branch-free coder: the wave's codestream window (both halves' regions) and the
+  // byte offset of this lane's region in it
+  __amdgpu_buffer_rsrc_t cs;
+  u32 cs_off;
 };
+// Buffer offset past every window: a store there is dropped, a load reads 0.
+constexpr u32 CS_DISCARD = 0xFFFFFFF0u;
 
 // Symbol-count mode sink: one symbol coded with the probability of LUT entry idx.
 __device__ __forceinline__ void tally(unsigned long long* hist, int idx, u32 sym) {
@@ -162,6 +168,34 @@ __device__ __forceinline__ void enc_symbol(LaneState<MAXP>& s, u32 sym, u32 prob
   if (s.Sz == 0) dst[g.region + s.slot] = (int32_t)s.L;
 }
 
+// Branch-free forms of the two coder steps, for a wave that is the critical path
+// of its launch: every lane computes the step and the lanes at the call site keep
+// it by select, so no exec-mask branch splits the scan step.  A closed codeword is
+// stored through the codestream window; the other lanes' stores fall outside the
+// window and are dropped.
+template <int MAXP>
+__device__ __forceinline__ void enc_sel(LaneState<MAXP>& s, bool at, u32 sym, u32 prob, const Geo& g) {
+  const u32 t = __umul24(s.Sz, prob) >> 7;
+  const u32 aux = t + sym;
+  const u32 nsz = sym ? s.Sz - aux : t;
+  const u32 nl = mad24(aux, sym, s.L);
+  s.Sz = at ? nsz : s.Sz;
+  s.L = at ? nl : s.L;
+  const bool close = at && nsz == 0;
+  __builtin_amdgcn_raw_buffer_store_b32(s.L, g.cs, close ? g.cs_off + s.slot * 4u : CS_DISCARD, 0, 0);
+}
+template <int MAXP>
+__device__ __forceinline__ u32 dec_sel(LaneState<MAXP>& s, bool at, u32 prob) {
+  const u32 aux = (__umul24(s.Sz, prob) >> 7) + 1u;
+  const u32 aux2 = s.L + aux;
+  const bool one = s.cw >= aux2;
+  const u32 nsz = one ? s.Sz - aux : aux - 1u;
+  const u32 nl = one ? aux2 : s.L;
+  s.Sz = at ? nsz : s.Sz;
+  s.L = at ? nl : s.L;
+  return (at && one) ? 1u : 0u;
+}
+
 // Decoder coder step (BPC_PaCo.cu:321-337).
 template <int MAXP>
 __device__ __forceinline__ u32 dec_symbol(LaneState<MAXP>& s, u32 prob) {
@@ -204,7 +238,7 @@ __device__ __forceinline__ u32 window(u64 x, int i) {
 // One scan step of the significance pass (CLEAN = 0, SPP_Encoder/SPP_Decoder,
 // BPC_PaCo.cu:412-507) or of the cleanup pass (CLEAN = 1, CP_Encoder/CP_Decoder,
 // BPC_PaCo.cu:510-589) at row i (in word H, window form W) of column ODD.
-template <int ODD, int H, int W, int DEC, int CP3, int CLEAN, int TRAIN, int MAXP>
+template <int ODD, int H, int W, int DEC, int CP3, int CLEAN, int TRAIN, int BF, int MAXP>
 __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u64 plane, const PlaneLut& pl,
                                          const Geo& g, const int32_t* src, int32_t* dst) {
   constexpr int c = ODD, o = 1 - ODD;
@@ -241,13 +275,15 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
   if (!DEC) {
     const u32 pbit = (word<H>(plane) >> ib) & 1u;  // the symbol, known before coding
     sym = at ? pbit : 0u;
-    if (at) {
-      if (TRAIN) {
-        if (pl.tal) tally(g.hist, (CLEAN ? pl.isig + pl.icp : pl.isig) + (int)ctx, pbit);
-      } else {
-        enc_symbol(s, pbit, (u32)((CLEAN ? pl.cpsig : pl.sig) >> (7u * ctx)) & 127u, g, dst);
-      }
+    if (TRAIN) {
+      if (at && pl.tal) tally(g.hist, (CLEAN ? pl.isig + pl.icp : pl.isig) + (int)ctx, pbit);
+    } else if (BF) {
+      enc_sel(s, at, pbit, (u32)((CLEAN ? pl.cpsig : pl.sig) >> (7u * ctx)) & 127u, g);
+    } else if (at) {
+      enc_symbol(s, pbit, (u32)((CLEAN ? pl.cpsig : pl.sig) >> (7u * ctx)) & 127u, g, dst);
     }
+  } else if (BF) {
+    sym = dec_sel(s, at, (u32)((CLEAN ? pl.cpsig : pl.sig) >> (7u * ctx)) & 127u);
   } else if (at) {
     sym = dec_symbol(s, (u32)((CLEAN ? pl.cpsig : pl.sig) >> (7u * ctx)) & 127u);
   }
@@ -274,7 +310,16 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
     const u32 hs = (PAIR_SIGN_TABLE >> (2 * hidx)) & 3u;
     const u32 sctx = (u32)(SIGN_CTX_TABLE >> (4 * (3 * hs + vs))) & 0xFu;
     if (!TRAIN) reserve<DEC>(s, nsig, g, src);
-    if (nsig) {
+    if (BF && !TRAIN) {
+      const u32 prob = ((CLEAN ? pl.cpsign : pl.sign) >> (8u * (sctx >> 1))) & 127u;
+      const u32 own = (word<H>(s.G[c]) >> ib) & 1u;
+      if (!DEC) {
+        enc_sel(s, nsig, own ^ (sctx & 1u), prob, g);
+      } else {
+        const u32 v = dec_sel(s, nsig, prob) ^ (sctx & 1u);
+        set_bit<H>(s.G[c], (nsig ? v : 0u) << ib);
+      }
+    } else if (nsig) {
       const u32 prob = ((CLEAN ? pl.cpsign : pl.sign) >> (8u * (sctx >> 1))) & 127u;
       const u32 own = (word<H>(s.G[c]) >> ib) & 1u;
       if (TRAIN) {
@@ -294,14 +339,19 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
 // "significant without refine flag -> set it" (BPC_PaCo.cu:604) touches only the
 // step's own coefficient, whose flag no later step of the pass reads, so the pass
 // applies it to all coefficients at its end (ref_pass).
-template <int ODD, int H, int DEC, int TRAIN, int MAXP>
+template <int ODD, int H, int DEC, int TRAIN, int BF, int MAXP>
 __device__ __forceinline__ void ref_step(LaneState<MAXP>& s, int i, bool act, u64 plane, const PlaneLut& pl,
                                          const Geo& g, const int32_t* src, int32_t* dst) {
   constexpr int c = ODD;
   const int ib = i - 32 * H;
   const bool at = act && ((word<H>(s.R[c]) >> ib) & 1u);
   if (!TRAIN) reserve<DEC>(s, at, g, src);
-  if (at) {
+  if (BF && !TRAIN) {
+    if (!DEC)
+      enc_sel(s, at, (word<H>(plane) >> ib) & 1u, pl.ref, g);
+    else
+      set_bit<H>(s.cur[c], dec_sel(s, at, pl.ref) << ib);
+  } else if (at) {
     if (TRAIN) {
       if (pl.tal) tally(g.hist, pl.iref, (word<H>(plane) >> ib) & 1u);
     } else if (!DEC) {
@@ -343,7 +393,7 @@ __device__ __forceinline__ u64 wave_or(u64 x) {
 // are not significant (significance only grows, one coefficient per step); CP
 // only those with the cleanup flag (never set during CP); MRP only those with the
 // refine flag (never set during MRP).  Rows 0-31 and 32-63 run as two loops.
-template <int DEC, int CP3, int CLEAN, int TRAIN, int MAXP>
+template <int DEC, int CP3, int CLEAN, int TRAIN, int BF, int MAXP>
 __device__ __forceinline__ void sig_pass(LaneState<MAXP>& s, bool act, const u64* plane, const PlaneLut& pl,
                                          const Geo& g, const int32_t* src, int32_t* dst) {
   const u64 rowmask = g.rows >= 64 ? ~0ull : ((1ull << g.rows) - 1ull);
@@ -351,26 +401,26 @@ __device__ __forceinline__ void sig_pass(LaneState<MAXP>& s, bool act, const u64
   const u64 mO = wave_or(act ? (CLEAN ? s.C[1] : (~s.S[1] & rowmask)) : 0ull);
   const u32 eL = (u32)mE, oL = (u32)mO, eH = (u32)(mE >> 32), oH = (u32)(mO >> 32);
   if ((eL | oL) & 1u) {
-    if (eL & 1u) sig_step<0, 0, 0, DEC, CP3, CLEAN, TRAIN>(s, 0, act, plane[0], pl, g, src, dst);
-    if (oL & 1u) sig_step<1, 0, 0, DEC, CP3, CLEAN, TRAIN>(s, 0, act, plane[1], pl, g, src, dst);
+    if (eL & 1u) sig_step<0, 0, 0, DEC, CP3, CLEAN, TRAIN, BF>(s, 0, act, plane[0], pl, g, src, dst);
+    if (oL & 1u) sig_step<1, 0, 0, DEC, CP3, CLEAN, TRAIN, BF>(s, 0, act, plane[1], pl, g, src, dst);
   }
   for (u32 r = (eL | oL) & ~1u; r; r &= r - 1u) {
     const int i = __builtin_ctz(r);
-    if ((eL >> i) & 1u) sig_step<0, 0, 1, DEC, CP3, CLEAN, TRAIN>(s, i, act, plane[0], pl, g, src, dst);
-    if ((oL >> i) & 1u) sig_step<1, 0, 1, DEC, CP3, CLEAN, TRAIN>(s, i, act, plane[1], pl, g, src, dst);
+    if ((eL >> i) & 1u) sig_step<0, 0, 1, DEC, CP3, CLEAN, TRAIN, BF>(s, i, act, plane[0], pl, g, src, dst);
+    if ((oL >> i) & 1u) sig_step<1, 0, 1, DEC, CP3, CLEAN, TRAIN, BF>(s, i, act, plane[1], pl, g, src, dst);
   }
   if ((eH | oH) & 1u) {
-    if (eH & 1u) sig_step<0, 1, 1, DEC, CP3, CLEAN, TRAIN>(s, 32, act, plane[0], pl, g, src, dst);
-    if (oH & 1u) sig_step<1, 1, 1, DEC, CP3, CLEAN, TRAIN>(s, 32, act, plane[1], pl, g, src, dst);
+    if (eH & 1u) sig_step<0, 1, 1, DEC, CP3, CLEAN, TRAIN, BF>(s, 32, act, plane[0], pl, g, src, dst);
+    if (oH & 1u) sig_step<1, 1, 1, DEC, CP3, CLEAN, TRAIN, BF>(s, 32, act, plane[1], pl, g, src, dst);
   }
   for (u32 r = (eH | oH) & ~1u; r; r &= r - 1u) {
     const int i = 32 + __builtin_ctz(r);
-    if ((eH >> (i - 32)) & 1u) sig_step<0, 1, 2, DEC, CP3, CLEAN, TRAIN>(s, i, act, plane[0], pl, g, src, dst);
-    if ((oH >> (i - 32)) & 1u) sig_step<1, 1, 2, DEC, CP3, CLEAN, TRAIN>(s, i, act, plane[1], pl, g, src, dst);
+    if ((eH >> (i - 32)) & 1u) sig_step<0, 1, 2, DEC, CP3, CLEAN, TRAIN, BF>(s, i, act, plane[0], pl, g, src, dst);
+    if ((oH >> (i - 32)) & 1u) sig_step<1, 1, 2, DEC, CP3, CLEAN, TRAIN, BF>(s, i, act, plane[1], pl, g, src, dst);
   }
 }
 
-template <int DEC, int TRAIN, int MAXP>
+template <int DEC, int TRAIN, int BF, int MAXP>
 __device__ __forceinline__ void ref_pass(LaneState<MAXP>& s, bool act, const u64* plane, const PlaneLut& pl,
                                          const Geo& g, const int32_t* src, int32_t* dst) {
   const u64 mE = wave_or(act ? s.R[0] : 0ull);  // rows with a refinement symbol in some lane
@@ -378,13 +428,13 @@ __device__ __forceinline__ void ref_pass(LaneState<MAXP>& s, bool act, const u64
   const u32 eL = (u32)mE, oL = (u32)mO, eH = (u32)(mE >> 32), oH = (u32)(mO >> 32);
   for (u32 r = eL | oL; r; r &= r - 1u) {
     const int i = __builtin_ctz(r);
-    if ((eL >> i) & 1u) ref_step<0, 0, DEC, TRAIN>(s, i, act, plane[0], pl, g, src, dst);
-    if ((oL >> i) & 1u) ref_step<1, 0, DEC, TRAIN>(s, i, act, plane[1], pl, g, src, dst);
+    if ((eL >> i) & 1u) ref_step<0, 0, DEC, TRAIN, BF>(s, i, act, plane[0], pl, g, src, dst);
+    if ((oL >> i) & 1u) ref_step<1, 0, DEC, TRAIN, BF>(s, i, act, plane[1], pl, g, src, dst);
   }
   for (u32 r = eH | oH; r; r &= r - 1u) {
     const int j = __builtin_ctz(r);
-    if ((eH >> j) & 1u) ref_step<0, 1, DEC, TRAIN>(s, 32 + j, act, plane[0], pl, g, src, dst);
-    if ((oH >> j) & 1u) ref_step<1, 1, DEC, TRAIN>(s, 32 + j, act, plane[1], pl, g, src, dst);
+    if ((eH >> j) & 1u) ref_step<0, 1, DEC, TRAIN, BF>(s, 32 + j, act, plane[0], pl, g, src, dst);
+    if ((oH >> j) & 1u) ref_step<1, 1, DEC, TRAIN, BF>(s, 32 + j, act, plane[1], pl, g, src, dst);
   }
   if (act) {  // every significant coefficient refines from the next plane on (BPC_PaCo.cu:604)
     s.R[0] |= s.S[0];
@@ -514,10 +564,11 @@ __device__ __forceinline__ u32 read_planes(const int32_t* src32, int64_t cbase, 
   return mor;
 }
 
+// BF = 1: the branch-free coder steps (enc_sel / dec_sel).
 // TRAIN = 1 (with DEC = 0): symbol-count mode.  The encoder's modelling runs
 // unchanged, but every symbol is counted at the LUT entry whose probability would
 // code it (hist, in LDS) instead of being coded; nothing is written to dst.
-template <int DEC, int CP3, int MAXP, int TRAIN = 0>
+template <int DEC, int CP3, int MAXP, int TRAIN = 0, int BF = 0>
 __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uint32_t* status_word, void* scr_base,
                                           unsigned long long* hist = nullptr) {
   const int lane = threadIdx.x & 63;
@@ -539,6 +590,17 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
   g.edge_even = hl == 0 ? 0u : 7u;
   g.edge_odd = hl == 31 ? 0u : 7u;
   g.hist = hist;
+  if (BF) {
+    // the window spans both halves' regions (two codeblocks of one image)
+    const bool v0 = __builtin_amdgcn_readlane((int)valid, 0), v1 = __builtin_amdgcn_readlane((int)valid, 32);
+    const int64_t q0 = (int64_t)__builtin_amdgcn_readlane(cbc, 0) * a.stride;
+    const int64_t q1 = (int64_t)__builtin_amdgcn_readlane(cbc, 32) * a.stride;
+    const int64_t lo = !v0 ? q1 : (!v1 ? q0 : min(q0, q1));
+    const int64_t hi = !v0 ? q1 : (!v1 ? q0 : max(q0, q1));
+    int32_t* base = (DEC ? const_cast<int32_t*>(a.src) : a.dst) + lo;
+    g.cs = __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)((hi - lo + a.stride) * 4), BUFFER_RSRC_WORD3);
+    g.cs_off = (u32)((g.region - lo) * 4);
+  }
   int64_t cbase;
   int rs;
   if (a.layout) {
@@ -647,10 +709,10 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
     if (!DEC && bp > 0) scr.plane(bp - 1, next);
     const bool act_sr = act && !(CP3 && bp == msb);
     if (ballot(act_sr)) {
-      sig_pass<DEC, CP3, 0, TRAIN>(s, act_sr, plane, pl, g, a.src, a.dst);
-      ref_pass<DEC, TRAIN>(s, act_sr, plane, pl, g, a.src, a.dst);
+      sig_pass<DEC, CP3, 0, TRAIN, BF>(s, act_sr, plane, pl, g, a.src, a.dst);
+      ref_pass<DEC, TRAIN, BF>(s, act_sr, plane, pl, g, a.src, a.dst);
     }
-    if (CP3) sig_pass<DEC, CP3, 1, TRAIN>(s, act, plane, pl, g, a.src, a.dst);
+    if (CP3) sig_pass<DEC, CP3, 1, TRAIN, BF>(s, act, plane, pl, g, a.src, a.dst);
     if (!DEC) {
       plane[0] = next[0];
       plane[1] = next[1];
@@ -733,11 +795,13 @@ __global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
   d.layout = a.layout;
   d.fill = a.fill;
   d.in16 = a.in16;
-  const int npairs = (a.ncb + 1) / 2;
-  for (int pair = wave; pair < npairs; pair += nwaves) {
-    const int cb = 2 * pair + ((threadIdx.x & 63) >> 5);
+  const int half = (threadIdx.x & 63) >> 5;
+  const int nunits = a.single ? a.ncb : (a.ncb + 1) / 2;
+  for (int unit = wave; unit < nunits; unit += nwaves) {
+    const int cb = a.single ? (half ? a.ncb : unit) : 2 * unit + half;
     d.cb = cb < a.ncb ? cb : -1;
-    code_pair<DEC, CP3, MAXP>(d, a.rows, a.status, scr);
+    // every wave of this launch may be the critical path: branch-free coder steps
+    code_pair<DEC, CP3, MAXP, 0, 1>(d, a.rows, a.status, scr);
   }
 }
 

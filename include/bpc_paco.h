@@ -46,8 +46,9 @@ typedef struct bpc_params {
   int32_t dwt_levels;      /* DWT_LEVELS (common.h:19). Default 5 */
   int32_t cblock_length;   /* CBLOCK_LENGTH (common.h:45): codeblock rows, 2..64. Default 64 */
   int32_t lut_n_bitplanes; /* LUT_N_BITPLANES (common.h:28): LUT rows per group. Default 15 */
-  int32_t cb_stride_words; /* int32 words per codeblock region; 0 = 64*cblock_length, the reference's
-                              layout (BPC_PaCo.cu:1130), in which the codestream has the image's size */
+  int32_t cb_stride_words; /* int32 words per codeblock region, 2..2^26; 0 = 64*cblock_length, the
+                              reference's layout (BPC_PaCo.cu:1130), in which the codestream has the
+                              image's size */
   int32_t layout;          /* BPC_LAYOUT_RASTER (default) or BPC_LAYOUT_TILED */
 } bpc_params;
 
