@@ -28,6 +28,13 @@ namespace {
 
 typedef uint64_t u64;
 typedef uint32_t u32;
+// Image, codestream and LUT pointers in the global address space: the batched
+// kernel reads them from its image table, where the compiler cannot tell them
+// from LDS or scratch pointers and would emit flat accesses.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+typedef gptr<int32_t> gi32;
+typedef gptr<const int32_t> cgi32;
 
 constexpr int N_SIG = 9;   // N_CONTEXT_SIGNIFICANCE (common.h:33)
 constexpr int N_SIGN = 4;  // N_CONTEXT_SIGN (common.h:32)
@@ -126,7 +133,7 @@ __device__ __forceinline__ void tally(unsigned long long* hist, int idx, u32 sym
 // order after the half's counter; the counter advances by their number.  The
 // counters stay wave-uniform (SGPRs).
 template <int DEC, int MAXP>
-__device__ __forceinline__ void reserve(LaneState<MAXP>& s, bool at, const Geo& g, const int32_t* src) {
+__device__ __forceinline__ void reserve(LaneState<MAXP>& s, bool at, const Geo& g, cgi32 src) {
   // 0 exactly for the lanes that reserve: one compare feeds the ballot directly
   const u32 szr = opaque(at ? s.Sz : 1u);
   const u64 m = ballot(szr == 0);
@@ -160,7 +167,7 @@ __device__ __forceinline__ u32 mad24(u32 a, u32 b, u32 c) {
 
 // Encoder coder step (BPC_PaCo.cu:281-294).
 template <int MAXP>
-__device__ __forceinline__ void enc_symbol(LaneState<MAXP>& s, u32 sym, u32 prob, const Geo& g, int32_t* dst) {
+__device__ __forceinline__ void enc_symbol(LaneState<MAXP>& s, u32 sym, u32 prob, const Geo& g, gi32 dst) {
   const u32 t = __umul24(s.Sz, prob) >> 7;  // Sz < 2^16, prob < 2^7
   const u32 aux = t + sym;                   // sym is 0 or 1
   s.Sz = sym ? s.Sz - aux : t;
@@ -240,7 +247,7 @@ __device__ __forceinline__ u32 window(u64 x, int i) {
 // BPC_PaCo.cu:510-589) at row i (in word H, window form W) of column ODD.
 template <int ODD, int H, int W, int DEC, int CP3, int CLEAN, int TRAIN, int BF, int MAXP>
 __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u64 plane, const PlaneLut& pl,
-                                         const Geo& g, const int32_t* src, int32_t* dst) {
+                                         const Geo& g, cgi32 src, gi32 dst) {
   constexpr int c = ODD, o = 1 - ODD;
   const int ib = i - 32 * H;
   const u32 bit = 1u << ib;
@@ -341,7 +348,7 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
 // applies it to all coefficients at its end (ref_pass).
 template <int ODD, int H, int DEC, int TRAIN, int BF, int MAXP>
 __device__ __forceinline__ void ref_step(LaneState<MAXP>& s, int i, bool act, u64 plane, const PlaneLut& pl,
-                                         const Geo& g, const int32_t* src, int32_t* dst) {
+                                         const Geo& g, cgi32 src, gi32 dst) {
   constexpr int c = ODD;
   const int ib = i - 32 * H;
   const bool at = act && ((word<H>(s.R[c]) >> ib) & 1u);
@@ -395,7 +402,7 @@ __device__ __forceinline__ u64 wave_or(u64 x) {
 // refine flag (never set during MRP).  Rows 0-31 and 32-63 run as two loops.
 template <int DEC, int CP3, int CLEAN, int TRAIN, int BF, int MAXP>
 __device__ __forceinline__ void sig_pass(LaneState<MAXP>& s, bool act, const u64* plane, const PlaneLut& pl,
-                                         const Geo& g, const int32_t* src, int32_t* dst) {
+                                         const Geo& g, cgi32 src, gi32 dst) {
   const u64 rowmask = g.rows >= 64 ? ~0ull : ((1ull << g.rows) - 1ull);
   const u64 mE = wave_or(act ? (CLEAN ? s.C[0] : (~s.S[0] & rowmask)) : 0ull);
   const u64 mO = wave_or(act ? (CLEAN ? s.C[1] : (~s.S[1] & rowmask)) : 0ull);
@@ -422,7 +429,7 @@ __device__ __forceinline__ void sig_pass(LaneState<MAXP>& s, bool act, const u64
 
 template <int DEC, int TRAIN, int BF, int MAXP>
 __device__ __forceinline__ void ref_pass(LaneState<MAXP>& s, bool act, const u64* plane, const PlaneLut& pl,
-                                         const Geo& g, const int32_t* src, int32_t* dst) {
+                                         const Geo& g, cgi32 src, gi32 dst) {
   const u64 mE = wave_or(act ? s.R[0] : 0ull);  // rows with a refinement symbol in some lane
   const u64 mO = wave_or(act ? s.R[1] : 0ull);
   const u32 eL = (u32)mE, oL = (u32)mO, eH = (u32)(mE >> 32), oH = (u32)(mO >> 32);
@@ -442,14 +449,14 @@ __device__ __forceinline__ void ref_pass(LaneState<MAXP>& s, bool act, const u64
   }
 }
 
-__device__ __forceinline__ u32 lut_at(const int32_t* lut, int idx, bool& bad) {
+__device__ __forceinline__ u32 lut_at(cgi32 lut, int idx, bool& bad) {
   const int32_t v = lut[idx];
   bad |= (u32)v > 127u;
   return (u32)v & 127u;
 }
 
 template <int CP3>
-__device__ __forceinline__ PlaneLut load_plane_lut(const int32_t* lut, int ref0, int sig0, int sign0, int cpoff,
+__device__ __forceinline__ PlaneLut load_plane_lut(cgi32 lut, int ref0, int sig0, int sign0, int cpoff,
                                                    int bp, bool& bad) {
   PlaneLut pl;
   pl.sig = 0;
@@ -473,10 +480,10 @@ __device__ __forceinline__ PlaneLut load_plane_lut(const int32_t* lut, int ref0,
 // The image one half-wave codes: wave-uniform in the per-image kernel,
 // half-uniform in the batched kernel (the halves may belong to different images).
 struct HalfDesc {
-  const int32_t* src;
-  int32_t* dst;
-  const int32_t* lut;
-  int32_t* ncw;
+  cgi32 src;
+  gi32 dst;
+  cgi32 lut;
+  gi32 ncw;
   int size_x, size_y, nx, stride, nbp, levels, lut_len, layout, fill;
   int in16;  // coefficients as int16 instead of int32
   int cb;    // codeblock of this half in its image, -1 for an empty half
@@ -517,9 +524,9 @@ struct Scratch {
 // Coefficients of one lane's stripe (int32 or int16, two's complement) into the
 // wave's bitplane scratch and the sign masks; returns the OR of the magnitudes.
 template <typename T, int MAXP>
-__device__ __forceinline__ u32 read_planes(const int32_t* src32, int64_t cbase, int rs, int rows, bool valid,
+__device__ __forceinline__ u32 read_planes(cgi32 src32, int64_t cbase, int rs, int rows, bool valid,
                                            const Scratch<MAXP>& scr, LaneState<MAXP>& s) {
-  const T* src = reinterpret_cast<const T*>(src32);
+  const gptr<const T> src = reinterpret_cast<gptr<const T>>(src32);
   u32 mor = 0;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
@@ -532,15 +539,15 @@ __device__ __forceinline__ u32 read_planes(const int32_t* src32, int64_t cbase, 
     for (int r = r0; r < r1; ++r) {
       int v0 = 0, v1 = 0;
       if (valid) {
-        const T* q = src + cbase + (int64_t)r * rs;
+        const gptr<const T> q = src + cbase + (int64_t)r * rs;
         if (sizeof(T) == 2) {
-          const short2 w = *reinterpret_cast<const short2*>(q);
-          v0 = w.x;
-          v1 = w.y;
+          const u32 w = *reinterpret_cast<gptr<const u32>>(q);
+          v0 = (int16_t)(w & 0xFFFFu);
+          v1 = (int16_t)(w >> 16);
         } else {
-          const int2 w = *reinterpret_cast<const int2*>(q);
-          v0 = w.x;
-          v1 = w.y;
+          const u64 w = *reinterpret_cast<gptr<const u64>>(q);
+          v0 = (int32_t)(u32)w;
+          v1 = (int32_t)(u32)(w >> 32);
         }
       }
       const u32 m0 = v0 < 0 ? 0u - (u32)v0 : (u32)v0;
@@ -597,7 +604,7 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
     const int64_t q1 = (int64_t)__builtin_amdgcn_readlane(cbc, 32) * a.stride;
     const int64_t lo = !v0 ? q1 : (!v1 ? q0 : min(q0, q1));
     const int64_t hi = !v0 ? q1 : (!v1 ? q0 : max(q0, q1));
-    int32_t* base = (DEC ? const_cast<int32_t*>(a.src) : a.dst) + lo;
+    int32_t* base = (int32_t*)(DEC ? const_cast<gi32>(a.src) : a.dst) + lo;
     g.cs = __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)((hi - lo + a.stride) * 4), BUFFER_RSRC_WORD3);
     g.cs_off = (u32)((g.region - lo) * 4);
   }
@@ -762,10 +769,10 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
             // int16 output: out-of-range values saturate and are reported
             const int s0 = max(-32768, min(32767, v0)), s1 = max(-32768, min(32767, v1));
             if (s0 != v0 || s1 != v1) status |= ST_NARROW;
-            *reinterpret_cast<short2*>(reinterpret_cast<int16_t*>(a.dst) + cbase + (int64_t)r * rs) =
-                make_short2((short)s0, (short)s1);
+            *reinterpret_cast<gptr<u32>>(reinterpret_cast<gptr<int16_t>>(a.dst) + cbase + (int64_t)r * rs) =
+                ((u32)s0 & 0xFFFFu) | ((u32)s1 << 16);
           } else {
-            *reinterpret_cast<int2*>(a.dst + cbase + (int64_t)r * rs) = make_int2(v0, v1);
+            *reinterpret_cast<gptr<u64>>(a.dst + cbase + (int64_t)r * rs) = (u64)(u32)v0 | ((u64)(u32)v1 << 32);
           }
         }
       }
@@ -781,10 +788,10 @@ __global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
   const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
   void* scr = reinterpret_cast<char*>(a.scratch) + (size_t)wave * (2 * MAXP * 64 * 8);
   HalfDesc d;
-  d.src = a.src;
-  d.dst = a.dst;
-  d.lut = a.lut;
-  d.ncw = a.ncw;
+  d.src = (cgi32)a.src;
+  d.dst = (gi32)a.dst;
+  d.lut = (cgi32)a.lut;
+  d.ncw = (gi32)a.ncw;
   d.size_x = a.size_x;
   d.size_y = a.size_y;
   d.nx = a.nx;
@@ -829,10 +836,10 @@ void bpc_batch_kernel(const BatchArgs a) {
     const u32 e = a.sorted[2 * (size_t)pair + half];
     const BatchImage& im = a.images[first >> 20];
     HalfDesc d;
-    d.src = im.src;
-    d.dst = im.dst;
-    d.lut = im.lut;
-    d.ncw = im.ncw;
+    d.src = (cgi32)im.src;
+    d.dst = (gi32)im.dst;
+    d.lut = (cgi32)im.lut;
+    d.ncw = (gi32)im.ncw;
     d.size_x = im.size_x;
     d.size_y = im.size_y;
     d.nx = im.nx;
@@ -865,7 +872,7 @@ __global__ __launch_bounds__(64 * TRAIN_WAVES) void bpc_train_kernel(const Train
   const int nwaves = (int)gridDim.x * TRAIN_WAVES;
   void* scr = reinterpret_cast<char*>(a.scratch) + (size_t)wave * scratch_bytes_per_wave(TRAIN_MAXP);
   HalfDesc d;
-  d.src = a.img;
+  d.src = (cgi32)a.img;
   d.dst = nullptr;
   d.lut = nullptr;
   d.ncw = nullptr;
