@@ -267,14 +267,14 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
     const bool cand = act && !(wc & 2u);
     if (CP3) {
       const bool gate = ctx != 0;  // BPC_PaCo.cu:424
-      if (cand && !gate) set_bit<H>(s.C[c], bit);
+      set_bit<H>(s.C[c], (cand && !gate ? 1u : 0u) << ib);  // BPC_PaCo.cu:424, without a branch
       at = cand && gate;
     } else {
       at = cand;
     }
   } else {
     at = act && ((word<H>(s.C[c]) >> ib) & 1u);  // BPC_PaCo.cu:515
-    if (at) clear_bit<H>(s.C[c], bit);            // BPC_PaCo.cu:528
+    clear_bit<H>(s.C[c], (at ? 1u : 0u) << ib);   // BPC_PaCo.cu:528, without a branch
   }
   // site 1: significance symbol
   if (!TRAIN) reserve<DEC>(s, at, g, src);
