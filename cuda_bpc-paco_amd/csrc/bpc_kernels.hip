@@ -326,17 +326,20 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
         const u32 v = dec_sel(s, nsig, prob) ^ (sctx & 1u);
         set_bit<H>(s.G[c], (nsig ? v : 0u) << ib);
       }
-    } else if (nsig) {
-      const u32 prob = ((CLEAN ? pl.cpsign : pl.sign) >> (8u * (sctx >> 1))) & 127u;
-      const u32 own = (word<H>(s.G[c]) >> ib) & 1u;
-      if (TRAIN) {
-        if (pl.tal) tally(g.hist, (CLEAN ? pl.isign + pl.icp : pl.isign) + (int)(sctx >> 1), own ^ (sctx & 1u));
-      } else if (!DEC) {
-        enc_symbol(s, own ^ (sctx & 1u), prob, g, dst);
-      } else {
-        const u32 v = dec_symbol(s, prob) ^ (sctx & 1u);
-        if (v) set_bit<H>(s.G[c], bit);
+    } else {
+      u32 v = 0;  // decoder: the sign bit, set outside the branch (no merge of the mask)
+      if (nsig) {
+        const u32 prob = ((CLEAN ? pl.cpsign : pl.sign) >> (8u * (sctx >> 1))) & 127u;
+        const u32 own = (word<H>(s.G[c]) >> ib) & 1u;
+        if (TRAIN) {
+          if (pl.tal) tally(g.hist, (CLEAN ? pl.isign + pl.icp : pl.isign) + (int)(sctx >> 1), own ^ (sctx & 1u));
+        } else if (!DEC) {
+          enc_symbol(s, own ^ (sctx & 1u), prob, g, dst);
+        } else {
+          v = dec_symbol(s, prob) ^ (sctx & 1u);
+        }
       }
+      if (DEC) set_bit<H>(s.G[c], v << ib);
     }
   }
 }
@@ -358,14 +361,18 @@ __device__ __forceinline__ void ref_step(LaneState<MAXP>& s, int i, bool act, u6
       enc_sel(s, at, (word<H>(plane) >> ib) & 1u, pl.ref, g);
     else
       set_bit<H>(s.cur[c], dec_sel(s, at, pl.ref) << ib);
-  } else if (at) {
-    if (TRAIN) {
-      if (pl.tal) tally(g.hist, pl.iref, (word<H>(plane) >> ib) & 1u);
-    } else if (!DEC) {
-      enc_symbol(s, (word<H>(plane) >> ib) & 1u, pl.ref, g, dst);
-    } else {
-      if (dec_symbol(s, pl.ref)) set_bit<H>(s.cur[c], 1u << ib);
+  } else {
+    u32 d = 0;  // decoder: the refinement bit, set outside the branch
+    if (at) {
+      if (TRAIN) {
+        if (pl.tal) tally(g.hist, pl.iref, (word<H>(plane) >> ib) & 1u);
+      } else if (!DEC) {
+        enc_symbol(s, (word<H>(plane) >> ib) & 1u, pl.ref, g, dst);
+      } else {
+        d = dec_symbol(s, pl.ref);
+      }
     }
+    if (DEC) set_bit<H>(s.cur[c], d << ib);
   }
 }
 
