@@ -11,6 +11,7 @@
 #include <atomic>
 #include <mutex>
 #include <thread>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -109,6 +110,21 @@ int device_cus(int dev) {
   return cache[dev];
 }
 
+// Resident waves per CU of a per-image kernel variant (an occupancy query, asked
+// once per device and variant).
+int resident_waves_cached(int dev, int decode, int cpass3, int maxp) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int, int>, int> cache;
+  const auto key = std::make_tuple(dev, decode, cpass3, maxp);
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int per_cu = bpc::codec_resident_waves_per_cu(decode, cpass3, maxp);
+  if (per_cu <= 0) per_cu = 1;
+  cache[key] = per_cu;
+  return per_cu;
+}
+
 // Bitplane scratch of the per-image kernel, one per device.  Launches that use it
 // are ordered through `last`: a launch on another stream first waits for the
 // previous user.
@@ -117,6 +133,7 @@ struct DevScratch {
   size_t bytes = 0;
   hipEvent_t last = nullptr;
   bool has_last = false;
+  hipStream_t last_stream = nullptr;  // stream of the previous user (same stream: already ordered)
 };
 std::mutex g_scratch_mu;
 std::vector<DevScratch> g_scratch;
@@ -134,7 +151,7 @@ int scratch_acquire(int dev, size_t need, hipStream_t stream, void** ptr) {
     sc.bytes = 0;
     HIP_TRY(hipMalloc(&sc.ptr, need));
     sc.bytes = need;
-  } else if (sc.has_last) {
+  } else if (sc.has_last && stream != sc.last_stream) {
     HIP_TRY(hipStreamWaitEvent(stream, sc.last, 0));
   }
   *ptr = sc.ptr;
@@ -145,6 +162,7 @@ int scratch_release(int dev, hipStream_t stream) {
   DevScratch& sc = g_scratch[dev];
   HIP_TRY(hipEventRecord(sc.last, stream));
   sc.has_last = true;
+  sc.last_stream = stream;
   return BPC_OK;
 }
 
@@ -172,8 +190,7 @@ int launch(int decode, const int32_t* src, int32_t* dst, int size_x, int size_y,
   if (a.ncb <= 0) return BPC_OK;
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
-  int per_cu = bpc::codec_resident_waves_per_cu(decode, p->cpass3, maxp);
-  if (per_cu <= 0) per_cu = 1;
+  int per_cu = resident_waves_cached(dev, decode, p->cpass3, maxp);
   // An image with no more codeblocks than the device has SIMDs cannot fill it with
   // pairs: one codeblock per wave then runs each codeblock's chain on its own SIMD
   // instead of two in lock step (the launch time is the longest chain).
