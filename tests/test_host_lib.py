@@ -53,6 +53,14 @@ def test_invalid_arguments_are_rejected(bpc):
     bad = bpc.params(cblock_length=1)
     assert lib.bpc_set_config(ctypes.byref(bad)) == bpc.ERR_ARG
     assert b"cblock_length" in lib.bpc_last_error()
+    # codeblock regions: 2 .. 2^26 words (the per-image kernel addresses a pair's regions with 32-bit offsets)
+    for stride in (1, (1 << 26) + 1):
+        bad = bpc.params(cb_stride_words=stride)
+        assert lib.bpc_set_config(ctypes.byref(bad)) == bpc.ERR_ARG
+        assert b"cb_stride_words" in lib.bpc_last_error()
+    ok = bpc.params(cb_stride_words=1 << 26)
+    assert lib.bpc_set_config(ctypes.byref(ok)) == bpc.OK
+    assert lib.bpc_set_config(ctypes.byref(p)) == bpc.OK
     # image not a whole number of codeblocks
     assert lib.bpc_run_ex(0, buf.ctypes.data, 100, 64, ctypes.c_void_p(8), 3360, ctypes.byref(p)) == bpc.ERR_ARG
     # LUT too short
