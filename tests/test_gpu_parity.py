@@ -93,6 +93,18 @@ def test_plain_4096_3pass(bpc, O):
     check_codec(bpc, O, img, lut, 1, nbp=16, layout=1)
 
 
+@pytest.mark.parametrize("size_y", [2048, 2112])
+def test_one_codeblock_per_wave_and_pairs(bpc, O, size_y):
+    """The per-image launch codes one codeblock per wave when the image has no more
+    codeblocks than the device has SIMDs (1024 on MI355X: 2048x2048), and pairs
+    above that (2048x2112 = 1056 codeblocks); both are the oracle's codestream."""
+    print("mode threshold", size_y, flush=True)
+    img = synth.synth_image(2048, size_y, 5, 230 + size_y, mag_bits=15)
+    p = bpc.params(cpass3=1, lut_n_bitplanes=15)
+    lut = bpc.train_lut(synth.synth_image(2048, size_y, 5, 231 + size_y, mag_bits=15), p)
+    check_codec(bpc, O, img, lut, 1, nbp=15)
+
+
 @pytest.mark.parametrize("cp3", [0, 1])
 def test_3072_straddling_subbands(bpc, O, cp3):
     """3072 = 96 * 32: the LL band is 96x96, so codeblocks straddle subbands and the
