@@ -27,6 +27,25 @@ constexpr int LP = W + 1;           // LDS row pitch (odd: column walks hit dist
 constexpr int THREADS = 256;        // four waves: lane tx owns a column (or a row), wave ty a residue mod 4
 constexpr int NR = W / 4;           // staged rows per thread
 
+// Workgroups are dealt to the XCDs round robin (workgroup w runs on XCD w % 8),
+// and each XCD has its own L2.  A tile's halo columns and band-edge samples share
+// 128-byte lines with its neighbours, so the launch gives each XCD one contiguous
+// run of tiles in raster order: horizontal (and most vertical) neighbours then
+// run on the same XCD, close in time, and their shared lines come from its L2
+// instead of HBM.  Returns false for the workgroups past the last tile.
+constexpr int XCDS = 8;
+
+__device__ __forceinline__ bool tile_of_workgroup(const DwtLevel& a, int& r0, int& c0) {
+  const int ntx = (a.nx + TILE - 1) / TILE, nty = (a.ny + TILE - 1) / TILE;
+  const int per = (ntx * nty + XCDS - 1) / XCDS;
+  const int w = blockIdx.x;
+  const int t = (w % XCDS) * per + w / XCDS;
+  if (t >= ntx * nty) return false;
+  r0 = (t / ntx) * TILE;
+  c0 = (t % ntx) * TILE;
+  return true;
+}
+
 // Whole-sample symmetric extension of index g into [0, n), n >= 2.
 __device__ __forceinline__ int reflect(int g, int n) {
   if ((unsigned)g < (unsigned)n) return g;
@@ -69,7 +88,8 @@ __device__ __forceinline__ void col_update(int32_t (*t)[LP], int c, int k0, int 
 
 __global__ __launch_bounds__(THREADS) void dwt53_forward_kernel(const DwtLevel a) {
   __shared__ int32_t t[W][LP];
-  const int r0 = blockIdx.y * TILE, c0 = blockIdx.x * TILE;
+  int r0, c0;
+  if (!tile_of_workgroup(a, r0, c0)) return;
   const Lane l;
   const int gc = reflect(c0 + l.tx, a.nx), ghc = reflect(c0 - HALO + l.hc, a.nx);
   // every load of the thread is issued before the first LDS store, so the tile
@@ -135,7 +155,8 @@ __global__ __launch_bounds__(THREADS) void dwt53_forward_kernel(const DwtLevel a
 
 __global__ __launch_bounds__(THREADS) void dwt53_inverse_kernel(const DwtLevel a) {
   __shared__ int32_t t[W][LP];
-  const int r0 = blockIdx.y * TILE, c0 = blockIdx.x * TILE;
+  int r0, c0;
+  if (!tile_of_workgroup(a, r0, c0)) return;
   const Lane l;
   const int hy = a.ny >> 1, hx = a.nx >> 1;
   // interleaved samples y (y[2n] = low band n, y[2n+1] = high band n) from the
@@ -200,7 +221,8 @@ __global__ __launch_bounds__(THREADS) void dwt53_inverse_kernel(const DwtLevel a
 
 hipError_t launch_dwt53_level(int inverse, const DwtLevel& a, hipStream_t stream) {
   if (a.nx < 2 || a.ny < 2 || (a.nx & 1) || (a.ny & 1)) return hipErrorInvalidValue;
-  const dim3 grid((a.nx + TILE - 1) / TILE, (a.ny + TILE - 1) / TILE);
+  const int tiles = ((a.nx + TILE - 1) / TILE) * ((a.ny + TILE - 1) / TILE);
+  const dim3 grid((tiles + XCDS - 1) / XCDS * XCDS);
   if (inverse)
     hipLaunchKernelGGL(dwt53_inverse_kernel, grid, dim3(THREADS), 0, stream, a);
   else
