@@ -7,7 +7,8 @@
 // array; offsets[] is the exclusive scan of the codeword counts.
 //
 // All four kernels are HBM-bound byte movers: one wave per codeblock, lanes
-// striding over the codewords (coalesced 256-byte reads, 128-byte writes).
+// striding over the codewords; regions move as 16-byte word groups when they are
+// 16-byte aligned.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -63,22 +64,35 @@ __global__ __launch_bounds__(64 * WAVES) void compact_index_kernel(const int32_t
 // each thread sums a contiguous chunk, the chunk sums are scanned in LDS, then each
 // thread rewrites its chunk as running sums.
 __global__ __launch_bounds__(1024) void compact_scan_kernel(u64* offsets, long long ncb) {
-  __shared__ u64 part[1024];
+  __shared__ u64 part[32];
   const int t = threadIdx.x;
   const long long chunk = (ncb + 1023) / 1024;
   const long long b = (long long)t * chunk;
   const long long e = b + chunk < ncb ? b + chunk : ncb;
   u64 sum = 0;
   for (long long k = b; k < e; ++k) sum += offsets[1 + k];
-  part[t] = sum;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const u64 v = t >= off ? part[t - off] : 0ull;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
+  // inclusive scan of the chunk sums: within each wave by shuffles, then over the
+  // 16 wave totals by the first wave (two barriers instead of twenty)
+  const int lane = t & 63, wid = t >> 6;
+  u64 x = sum;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const u64 y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
   }
-  u64 run = t ? part[t - 1] : 0ull;
+  if (lane == 63) part[wid] = x;
+  __syncthreads();
+  if (wid == 0) {
+    u64 v = lane < 16 ? part[lane] : 0ull;
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+      const u64 y = __shfl_up(v, off, 64);
+      if (lane >= off) v += y;
+    }
+    if (lane < 16) part[16 + lane] = v;
+  }
+  __syncthreads();
+  u64 run = x - sum + (wid ? part[16 + wid - 1] : 0ull);  // exclusive prefix of this thread's chunk
   for (long long k = b; k < e; ++k) {
     run += offsets[1 + k];
     offsets[1 + k] = run;
@@ -87,9 +101,11 @@ __global__ __launch_bounds__(1024) void compact_scan_kernel(u64* offsets, long l
 }
 
 // Codewords of every region into the payload (16 bits each, cap codewords).
+// vec: regions are 16-byte aligned (stream aligned, stride a multiple of 4 words)
+// and the payload 4-byte aligned.
 __global__ __launch_bounds__(64 * WAVES) void compact_pack_kernel(const int32_t* stream, long long ncb, int stride,
                                                                   const u64* offsets, uint16_t* payload, u64 cap,
-                                                                  uint32_t* status) {
+                                                                  uint32_t* status, int vec) {
   const long long cb = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   if (cb >= ncb) return;
   const int lane = threadIdx.x & 63;
@@ -99,16 +115,44 @@ __global__ __launch_bounds__(64 * WAVES) void compact_pack_kernel(const int32_t*
     if (lane == 0) atomicOr(status, ST_OVERFLOW);
     return;
   }
-  const int32_t* src = stream + cb * (long long)stride + 1;
-  uint16_t* dst = payload + o;
-  for (long long j = lane; j < n; j += 64) dst[j] = (uint16_t)src[j];
+  const int32_t* region = stream + cb * (long long)stride;
+  uint16_t* dst = payload + o;  // word w of the region (1 <= w <= n) goes to dst[w - 1]
+  if (vec) {
+    // 16-byte reads of word groups [4q, 4q + 4): a quarter of the memory round trips
+    const long long ng = (n + 4) / 4;  // groups over words 0..n
+    for (long long q = lane; q < ng; q += 64) {
+      const int4 v = *reinterpret_cast<const int4*>(region + 4 * q);
+      const long long w = 4 * q;
+      if (w >= 1 && w + 3 <= n) {
+        // a whole group: codeword pairs at even payload positions as 4-byte stores
+        // (o, hence the parity, is the same for the whole wave)
+        uint16_t* d = dst + (w - 1);
+        if (((o + (u64)w - 1) & 1) == 0) {
+          reinterpret_cast<uint32_t*>(d)[0] = ((uint32_t)v.x & 0xFFFFu) | ((uint32_t)v.y << 16);
+          reinterpret_cast<uint32_t*>(d)[1] = ((uint32_t)v.z & 0xFFFFu) | ((uint32_t)v.w << 16);
+        } else {
+          d[0] = (uint16_t)v.x;
+          *reinterpret_cast<uint32_t*>(d + 1) = ((uint32_t)v.y & 0xFFFFu) | ((uint32_t)v.z << 16);
+          d[3] = (uint16_t)v.w;
+        }
+      } else {
+        if (w >= 1 && w <= n) dst[w - 1] = (uint16_t)v.x;
+        if (w + 1 <= n) dst[w] = (uint16_t)v.y;
+        if (w + 2 <= n) dst[w + 1] = (uint16_t)v.z;
+        if (w + 3 <= n) dst[w + 2] = (uint16_t)v.w;
+      }
+    }
+  } else {
+    for (long long j = 1 + lane; j <= n; j += 64) dst[j - 1] = (uint16_t)region[j];
+  }
 }
 
 // Compact form back to reference regions: MSB word, codewords, -1 in the rest.
 // A codeblock whose count exceeds the region keeps only its MSB (overflow status).
 __global__ __launch_bounds__(64 * WAVES) void compact_unpack_kernel(const uint8_t* msb, const u64* offsets,
                                                                     const uint16_t* payload, u64 cap, long long ncb,
-                                                                    int stride, int32_t* stream, uint32_t* status) {
+                                                                    int stride, int32_t* stream, uint32_t* status,
+                                                                    int vec) {
   const long long cb = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   if (cb >= ncb) return;
   const int lane = threadIdx.x & 63;
@@ -119,9 +163,23 @@ __global__ __launch_bounds__(64 * WAVES) void compact_unpack_kernel(const uint8_
     n = 0;
   }
   int32_t* region = stream + cb * (long long)stride;
-  if (lane == 0) region[0] = msb[cb];
-  const uint16_t* src = payload + o;
-  for (long long j = lane; j < stride - 1; j += 64) region[1 + j] = j < n ? (int32_t)src[j] : -1;
+  const uint16_t* src = payload + o;  // region word w (1 <= w <= n) is src[w - 1]
+  if (vec) {
+    // 16-byte stores of word groups [4q, 4q + 4): a quarter of the store instructions
+    const int32_t m = msb[cb];
+    for (long long q = lane; q < stride / 4; q += 64) {
+      const long long w = 4 * q;
+      int4 v;
+      v.x = w == 0 ? m : (w <= n ? (int32_t)src[w - 1] : -1);
+      v.y = w + 1 <= n ? (int32_t)src[w] : -1;
+      v.z = w + 2 <= n ? (int32_t)src[w + 1] : -1;
+      v.w = w + 3 <= n ? (int32_t)src[w + 2] : -1;
+      *reinterpret_cast<int4*>(region + w) = v;
+    }
+  } else {
+    if (lane == 0) region[0] = msb[cb];
+    for (long long j = 1 + lane; j < stride; j += 64) region[j] = j <= n ? (int32_t)src[j - 1] : -1;
+  }
 }
 
 }  // namespace
@@ -142,8 +200,9 @@ hipError_t launch_compact_pack(const int32_t* stream, long long ncb, int stride,
                                uint16_t* payload, unsigned long long cap, uint32_t* status, hipStream_t s) {
   if (ncb <= 0) return hipSuccess;
   const long long blocks = (ncb + WAVES - 1) / WAVES;
+  const int vec = ((uintptr_t)stream % 16 == 0 && stride % 4 == 0 && (uintptr_t)payload % 4 == 0) ? 1 : 0;
   hipLaunchKernelGGL(compact_pack_kernel, dim3((unsigned)blocks), dim3(64 * WAVES), 0, s, stream, ncb, stride,
-                     offsets, payload, (u64)cap, status);
+                     offsets, payload, (u64)cap, status, vec);
   return hipGetLastError();
 }
 
@@ -152,8 +211,9 @@ hipError_t launch_compact_unpack(const uint8_t* msb, const unsigned long long* o
                                  hipStream_t s) {
   if (ncb <= 0) return hipSuccess;
   const long long blocks = (ncb + WAVES - 1) / WAVES;
+  const int vec = ((uintptr_t)stream % 16 == 0 && stride % 4 == 0) ? 1 : 0;
   hipLaunchKernelGGL(compact_unpack_kernel, dim3((unsigned)blocks), dim3(64 * WAVES), 0, s, msb, offsets, payload,
-                     (u64)cap, ncb, stride, stream, status);
+                     (u64)cap, ncb, stride, stream, status, vec);
   return hipGetLastError();
 }
 

@@ -24,8 +24,9 @@ def encoded(bpc, O, size, cp3, nbp, seed, rows=64, stride=0, mag_bits=14, zero_c
 
 @pytest.mark.parametrize("size,cp3,nbp,rows,stride,derive", [
     (2048, 0, 15, 64, 0, False), (2048, 0, 15, 64, 0, True), (1024, 1, 16, 32, 0, True),
-    (1024, 1, 15, 64, 6000, True), (4096, 1, 16, 64, 0, False)])
+    (1024, 1, 15, 64, 6000, True), (1024, 0, 15, 64, 6002, True), (4096, 1, 16, 64, 0, False)])
 def test_pack_unpack_matches_oracle(bpc, O, size, cp3, nbp, rows, stride, derive):
+    # regions that are multiples of 4 words move as 16-byte groups; 6002 takes the word path
     print("container", size, cp3, nbp, rows, stride, derive, flush=True)
     img, p, lut_arr, ostream, oncw = encoded(bpc, O, size, cp3, nbp, 140 + size // 1024 + cp3, rows, stride,
                                              mag_bits=16 if nbp >= 16 else 14)
