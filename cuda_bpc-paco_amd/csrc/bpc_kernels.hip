@@ -3,19 +3,21 @@
 // Work mapping.  The codestream format depends on 32 arithmetic coders per
 // 64-column codeblock (one per 2-column stripe, BPC_PaCo.cu:1124-1130), so one
 // wave64 carries two codeblocks: lanes 0-31 code codeblock 2w, lanes 32-63 code
-// codeblock 2w+1.  Lane l of a half owns columns 2(l%32) and 2(l%32)+1.
+// codeblock 2w+1 (the per-image kernel gives small images one codeblock per
+// wave).  Lane l of a half owns columns 2(l%32) and 2(l%32)+1.
 //
 // State layout.  Instead of the reference's 128 packed words per lane
 // (TData[], BPC_PaCo.cu:1120) each lane keeps its stripe bit-sliced: one 64-bit
 // row mask per column for each flag (significant S, refine R, cleanup C, sign),
 // so a context is a popcount of three 3-row windows and a flag update is one OR.
-// The encoder's magnitude bitplanes (and the decoder's decoded bitplanes) sit
-// in a register shift register of MAXP planes, so every index is static.
+// The encoder's magnitude bitplanes (and the decoder's decoded bitplanes) live in
+// a per-wave scratch slot, one plane (two row masks per lane) loaded or stored
+// per bitplane.
 //
-// Cross-lane traffic per scan step: one DPP wave shift of a packed word (the
-// neighbour column's 3-row significance window and its sign bit); codeword
-// slots: one 64-bit ballot per coder call site, ranked per 32-lane half with
-// mbcnt, and two wave-uniform (SGPR) slot counters, one per half.
+// Cross-lane traffic per scan step: one DPP wave shift of the neighbour column's
+// 3-row significance window (and, where a coefficient became significant, one of
+// its sign bit); codeword slots: one 64-bit ballot per coder call site, ranked per
+// 32-lane half with mbcnt, and two wave-uniform (SGPR) slot counters, one per half.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
