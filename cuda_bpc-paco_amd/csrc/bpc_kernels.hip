@@ -235,12 +235,14 @@ __device__ __forceinline__ void clear_bit(u64& x, u32 b) {
   x &= H ? ~((u64)b << 32) : ~(u64)b;
 }
 // 3-row window of a column mask: rows i-1, i, i+1 at bits 0..2 (rows -1 and 64
-// are 0).  W = 0: row 0; W = 1: rows 1..32 (one funnel shift across the words);
-// W = 2: rows 33..63 (one field extract from the upper word).
+// are 0).  W = 0: row 0; W = 1: rows 31 and 32 (one funnel shift across the
+// words); W = 2: rows 33..63 (one field extract from the upper word); W = 3: rows
+// 1..30 (one field extract from the lower word).
 template <int W>
 __device__ __forceinline__ u32 window(u64 x, int i) {
   if (W == 0) return ((u32)x << 1) & 7u;
   if (W == 1) return __builtin_amdgcn_alignbit((u32)(x >> 32), (u32)x, (u32)(i - 1)) & 7u;
+  if (W == 3) return __builtin_amdgcn_ubfe((u32)x, (u32)(i - 1), 3u);
   return __builtin_amdgcn_ubfe((u32)(x >> 32), (u32)(i - 33), 3u);
 }
 
@@ -420,10 +422,14 @@ __device__ __forceinline__ void sig_pass(LaneState<MAXP>& s, bool act, const u64
     if (eL & 1u) sig_step<0, 0, 0, DEC, CP3, CLEAN, TRAIN, BF>(s, 0, act, plane[0], pl, g, src, dst);
     if (oL & 1u) sig_step<1, 0, 0, DEC, CP3, CLEAN, TRAIN, BF>(s, 0, act, plane[1], pl, g, src, dst);
   }
-  for (u32 r = (eL | oL) & ~1u; r; r &= r - 1u) {
+  for (u32 r = (eL | oL) & 0x7FFFFFFEu; r; r &= r - 1u) {
     const int i = __builtin_ctz(r);
-    if ((eL >> i) & 1u) sig_step<0, 0, 1, DEC, CP3, CLEAN, TRAIN, BF>(s, i, act, plane[0], pl, g, src, dst);
-    if ((oL >> i) & 1u) sig_step<1, 0, 1, DEC, CP3, CLEAN, TRAIN, BF>(s, i, act, plane[1], pl, g, src, dst);
+    if ((eL >> i) & 1u) sig_step<0, 0, 3, DEC, CP3, CLEAN, TRAIN, BF>(s, i, act, plane[0], pl, g, src, dst);
+    if ((oL >> i) & 1u) sig_step<1, 0, 3, DEC, CP3, CLEAN, TRAIN, BF>(s, i, act, plane[1], pl, g, src, dst);
+  }
+  if ((eL | oL) >> 31) {  // row 31: its window reaches row 32 in the upper word
+    if (eL >> 31) sig_step<0, 0, 1, DEC, CP3, CLEAN, TRAIN, BF>(s, 31, act, plane[0], pl, g, src, dst);
+    if (oL >> 31) sig_step<1, 0, 1, DEC, CP3, CLEAN, TRAIN, BF>(s, 31, act, plane[1], pl, g, src, dst);
   }
   if ((eH | oH) & 1u) {
     if (eH & 1u) sig_step<0, 1, 1, DEC, CP3, CLEAN, TRAIN, BF>(s, 32, act, plane[0], pl, g, src, dst);
