@@ -506,6 +506,66 @@ def bench_dwt(args, check=True):
             "round_trip_exact": True, "bit_exact_vs_oracle": exact}
 
 
+def pinned_int32(n):
+    """A page-locked host array of n int32 (hipHostMalloc, as the reference example's
+    cudaMallocHost, main_example.cu:508), or None when the HIP runtime is not found."""
+    import ctypes
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError:
+        try:
+            hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+        except OSError:
+            return None, None
+    ptr = ctypes.c_void_p()
+    if hip.hipHostMalloc(ctypes.byref(ptr), ctypes.c_size_t(4 * n), ctypes.c_uint(0)) != 0 or not ptr.value:
+        return None, None
+    arr = np.ctypeslib.as_array((ctypes.c_int32 * n).from_address(ptr.value))
+
+    def free():
+        hip.hipHostFree(ptr)
+    return arr, free
+
+
+def bench_drop_in(args):
+    """The drop-in CUDA_BPC path end to end (BPC_PaCo.cu:1305-1325): host buffer in,
+    device buffers allocated, H2D copy, kernel, D2H copy, freed, per call; the plain
+    4096^2 3-pass image in raster layout, from pageable and from pinned host memory."""
+    size, cp3, nbp = 4096, 1, 16
+    log("drop_in: generating image and training LUT")
+    img, lut_arr, _ = make_workload(size, cp3, nbp, mag_bits=16, ll_bits=13, detail_exp=9, seed=args.seed)
+    p = bpc.params(cpass3=cp3, lut_n_bitplanes=nbp)
+    lut = bpc.Lut.from_array(lut_arr)
+    n = max(size * size, bpc.stream_words(size, size, p))
+    out = {"workload": "drop_in_4096_3pass (bpc_run: allocation, copies and kernel per call, raster layout)"}
+    pinned, free = pinned_int32(n)
+    kinds = [("pageable", np.zeros(n, np.int32))] + ([("pinned", pinned)] if pinned is not None else [])
+    try:
+        for kind, buf in kinds:
+            times = []
+            for rep in range(6):
+                buf[: size * size] = img.ravel()
+                t0 = time.perf_counter()
+                bpc.run(bpc.CODE, buf, size, size, lut, p)
+                t1 = time.perf_counter()
+                bpc.run(bpc.DECODE, buf, size, size, lut, p)
+                t2 = time.perf_counter()
+                if not np.array_equal(buf[: size * size].reshape(size, size), img):
+                    raise RuntimeError("drop-in round trip differs (%s)" % kind)
+                if rep:
+                    times.append((t1 - t0, t2 - t1))
+            e, d = np.median(times, axis=0)
+            out[kind] = {"encode_ms": round(e * 1e3, 3), "decode_ms": round(d * 1e3, 3),
+                         "msamples_s": round(size * size / (e + d) / 1e6, 1), "round_trip_exact": True}
+    finally:
+        del kinds
+        if free is not None:
+            del pinned
+            free()
+        lut.free()
+    return out
+
+
 def bench_int16(args):
     """int16 coefficient path: a 4096x4096 image of at most 15 magnitude bits, 3
     passes, coded from int16 and from int32 coefficients of equal value (the
@@ -605,8 +665,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle baseline and parity check")
-    ap.add_argument("--workloads", default="plain,batched,served,large,train,container,dwt,int16",
-                    help="comma list of plain, batched, served, large, train, container, dwt, int16 "
+    ap.add_argument("--workloads", default="plain,batched,served,large,train,container,dwt,int16,drop_in",
+                    help="comma list of plain, batched, served, large, train, container, dwt, int16, drop_in "
                          "(plain always runs: it gives `value`)")
     ap.add_argument("--vgpr-cap", type=int, default=8,
                     help="register cap of the batched kernel for the large workload (0 or 5..8; the batched "
@@ -756,6 +816,8 @@ def main():
         out["lut_trainer"] = bench_train(args)
     if "int16" in wl:
         out["int16"] = bench_int16(args)
+    if "drop_in" in wl:
+        out["drop_in"] = bench_drop_in(args)
     if "dwt" in wl:
         out["dwt53"] = bench_dwt(args, check=not args.no_cpu)
     print(json.dumps(out), flush=True)
