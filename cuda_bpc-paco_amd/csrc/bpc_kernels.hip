@@ -45,21 +45,28 @@ constexpr int N_SIGN = 4;  // N_CONTEXT_SIGN (common.h:32)
 // 3*hs + vs with hs, vs in {0: zero, 1: positive, 2: negative}.
 constexpr u64 SIGN_CTX_TABLE = 0x715064320ull;
 
-// Sign code {0: zero, 1: positive, 2: negative} of chi(a) + chi(b) for a pair of
-// neighbours, chi = 0 when not significant, -1 when negative, +1 otherwise
-// (BPC_PaCo.cu:220-225).  Index bits: 0 = a significant, 1 = a negative,
-// 2 = b significant, 3 = b negative (negative bits only with the significant bit).
-constexpr u32 pair_sign_table() {
-  u32 t = 0;
-  for (u32 k = 0; k < 16; ++k) {
-    const int ca = (k & 1u) ? ((k & 2u) ? -1 : 1) : 0;
-    const int cb = (k & 4u) ? ((k & 8u) ? -1 : 1) : 0;
-    const int sum = ca + cb;
-    t |= (u32)(sum > 0 ? 1 : (sum < 0 ? 2 : 0)) << (2 * k);
-  }
-  return t;
+// Sign context per 8-bit neighbour state, as an LDS table built at kernel start.
+// Index bits: 0 own other column's sign and 1 its significance (row i); 2 sign and
+// 3 significance of the row above, 4 sign and 5 significance of the row below (own
+// column); 6 sign and 7 significance of the neighbour lane's column (row i).  A sign
+// bit counts only with its significance bit.  Entry: the context (0..3) in bits 3-4,
+// the predicted-sign flip in bit 5 (chi = 0 not significant, -1 negative, +1
+// positive; h and v are the sums of the horizontal and vertical pairs,
+// BPC_PaCo.cu:178-226).
+__device__ __forceinline__ int chi(u32 sig, u32 neg) { return sig ? (neg ? -1 : 1) : 0; }
+__device__ __forceinline__ uint8_t sign_lut_entry(u32 k) {
+  const int h = chi((k >> 1) & 1u, k & 1u) + chi((k >> 7) & 1u, (k >> 6) & 1u);
+  const int v = chi((k >> 3) & 1u, (k >> 2) & 1u) + chi((k >> 5) & 1u, (k >> 4) & 1u);
+  const u32 hs = h > 0 ? 1u : (h < 0 ? 2u : 0u), vs = v > 0 ? 1u : (v < 0 ? 2u : 0u);
+  const u32 s4 = (u32)(SIGN_CTX_TABLE >> (4 * (3 * hs + vs))) & 0xFu;
+  return (uint8_t)(((s4 >> 1) << 3) | ((s4 & 1u) << 5));
 }
-constexpr u32 PAIR_SIGN_TABLE = pair_sign_table();
+typedef __attribute__((address_space(3))) const uint8_t* lds_u8;
+__device__ __forceinline__ lds_u8 build_sign_lut(uint8_t* t) {
+  for (u32 k = threadIdx.x; k < 256; k += blockDim.x) t[k] = sign_lut_entry(k);
+  __syncthreads();
+  return (lds_u8)t;
+}
 
 struct PlaneLut {  // probabilities of one bitplane, 7 bits each
   u64 sig;         // 9 significance contexts, entry k at bits 7k
@@ -117,6 +124,7 @@ struct Geo {
   u32 edge_even;   // 7, or 0 for the first lane of a half (no left neighbour in the codeblock)
   u32 edge_odd;    // 7, or 0 for the last lane of a half (no right neighbour)
   unsigned long long* hist;  // symbol-count mode: per LUT entry, (symbols << 32) + zeros, in LDS
+  lds_u8 slut;               // sign contexts (build_sign_lut)
   // branch-free coder: the wave's codestream window (both halves' regions) and the
   // byte offset of this lane's region in it
   __amdgpu_buffer_rsrc_t cs;
@@ -264,7 +272,8 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
   const u32 nb = (ODD ? dpp_from_right(wo) : dpp_from_left(wo)) & (ODD ? g.edge_odd : g.edge_even);
   // context = significant neighbours; wc includes the coefficient itself, which is
   // not significant wherever a context is used
-  const u32 ctx = __builtin_popcount(wo | (wc << 3) | (nb << 6));
+  const u32 nbr = wo | (wc << 3) | (nb << 6);
+  const u32 ctx = __builtin_popcount(nbr);
 
   bool at;
   if (!CLEAN) {
@@ -309,38 +318,35 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
     const bool nsig = sym != 0;
     // neighbour lane's sign bit at row i, moved like its significance window
     const u32 gs = (word<H>(s.G[o]) >> ib) & 1u;
-    const u32 gn = (ODD ? dpp_from_right(gs) : dpp_from_left(gs)) & ((nb >> 1) & 1u);
-    // (significant, negative) of the vertical pair U, D (own column rows i-1, i+1)
-    // and of the horizontal pair: neighbour lane's column and own other column at row i
-    const u32 gc = window<W>(s.G[c], i) & 5u;
-    const u32 wv = wc & 5u;
-    const u32 vidx = wv | ((gc & wv) << 1);
-    const u32 so = (wo >> 1) & 1u, go = gs & so;
-    const u32 hidx = ((nb >> 1) & 1u) | (gn << 1) | (so << 2) | (go << 3);
-    const u32 vs = (PAIR_SIGN_TABLE >> (2 * vidx)) & 3u;
-    const u32 hs = (PAIR_SIGN_TABLE >> (2 * hidx)) & 3u;
-    const u32 sctx = (u32)(SIGN_CTX_TABLE >> (4 * (3 * hs + vs))) & 0xFu;
+    const u32 gn = ODD ? dpp_from_right(gs) : dpp_from_left(gs);
+    // the 8 neighbour bits (build_sign_lut): significance bits straight from the
+    // context word, sign bits from the own column's sign window and the two row-i
+    // sign bits; the context and flip come from one LDS byte
+    const u32 idx = (nbr & 0xAAu) | ((window<W>(s.G[c], i) << 2) & 0x14u) | gs | (gn << 6);
+    const u32 e = g.slut[idx];
+    const u32 ssh = e & 24u;   // 8 x sign context: its probability's bit offset
+    const u32 flip = e >> 5;   // predicted sign
     if (!TRAIN) reserve<DEC>(s, nsig, g, src);
     if (BF && !TRAIN) {
-      const u32 prob = ((CLEAN ? pl.cpsign : pl.sign) >> (8u * (sctx >> 1))) & 127u;
+      const u32 prob = ((CLEAN ? pl.cpsign : pl.sign) >> ssh) & 127u;
       const u32 own = (word<H>(s.G[c]) >> ib) & 1u;
       if (!DEC) {
-        enc_sel(s, nsig, own ^ (sctx & 1u), prob, g);
+        enc_sel(s, nsig, own ^ flip, prob, g);
       } else {
-        const u32 v = dec_sel(s, nsig, prob) ^ (sctx & 1u);
+        const u32 v = dec_sel(s, nsig, prob) ^ flip;
         set_bit<H>(s.G[c], (nsig ? v : 0u) << ib);
       }
     } else {
       u32 v = 0;  // decoder: the sign bit, set outside the branch (no merge of the mask)
       if (nsig) {
-        const u32 prob = ((CLEAN ? pl.cpsign : pl.sign) >> (8u * (sctx >> 1))) & 127u;
+        const u32 prob = ((CLEAN ? pl.cpsign : pl.sign) >> ssh) & 127u;
         const u32 own = (word<H>(s.G[c]) >> ib) & 1u;
         if (TRAIN) {
-          if (pl.tal) tally(g.hist, (CLEAN ? pl.isign + pl.icp : pl.isign) + (int)(sctx >> 1), own ^ (sctx & 1u));
+          if (pl.tal) tally(g.hist, (CLEAN ? pl.isign + pl.icp : pl.isign) + (int)(ssh >> 3), own ^ flip);
         } else if (!DEC) {
-          enc_symbol(s, own ^ (sctx & 1u), prob, g, dst);
+          enc_symbol(s, own ^ flip, prob, g, dst);
         } else {
-          v = dec_symbol(s, prob) ^ (sctx & 1u);
+          v = dec_symbol(s, prob) ^ flip;
         }
       }
       if (DEC) set_bit<H>(s.G[c], v << ib);
@@ -502,6 +508,7 @@ struct HalfDesc {
   int size_x, size_y, nx, stride, nbp, levels, lut_len, layout, fill;
   int in16;  // coefficients as int16 instead of int32
   int cb;    // codeblock of this half in its image, -1 for an empty half
+  lds_u8 slut;  // sign contexts in LDS (build_sign_lut)
 };
 
 // Codes the two codeblocks of one wave (Kernel_BPC_CODER / Kernel_BPC_DECODER
@@ -612,6 +619,7 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
   g.edge_even = hl == 0 ? 0u : 7u;
   g.edge_odd = hl == 31 ? 0u : 7u;
   g.hist = hist;
+  g.slut = a.slut;
   if (BF) {
     // the window spans both halves' regions (two codeblocks of one image)
     const bool v0 = __builtin_amdgcn_readlane((int)valid, 0), v1 = __builtin_amdgcn_readlane((int)valid, 32);
@@ -799,6 +807,8 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
 
 template <int DEC, int CP3, int MAXP>
 __global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
+  __shared__ uint8_t slut[256];
+  const lds_u8 sl = build_sign_lut(slut);
   const int wave = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
   void* scr = reinterpret_cast<char*>(a.scratch) + (size_t)wave * (2 * MAXP * 64 * 8);
@@ -817,6 +827,7 @@ __global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
   d.layout = a.layout;
   d.fill = a.fill;
   d.in16 = a.in16;
+  d.slut = sl;
   const int half = (threadIdx.x & 63) >> 5;
   const int nunits = a.single ? a.ncb : (a.ncb + 1) / 2;
   for (int unit = wave; unit < nunits; unit += nwaves) {
@@ -838,6 +849,8 @@ __global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
 template <int DEC, int MAXP, int OCC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC ? OCC : 1, OCC ? OCC : 8)))
 void bpc_batch_kernel(const BatchArgs a) {
+  __shared__ uint8_t slut[256];
+  const lds_u8 sl = build_sign_lut(slut);
   const int half = (threadIdx.x & 63) >> 5;
   const size_t wave = (size_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   void* scr = reinterpret_cast<char*>(a.scratch) + wave * (2 * MAXP * 64 * 8);
@@ -865,6 +878,7 @@ void bpc_batch_kernel(const BatchArgs a) {
     d.layout = im.layout;
     d.fill = im.fill;
     d.in16 = 0;
+    d.slut = sl;
     d.cb = e == QUEUE_EMPTY ? -1 : (int)(e & 0xFFFFFu);
     if (im.cpass3)
       code_pair<DEC, 1, MAXP>(d, a.rows, &a.ctl[CTL_STATUS], scr);
@@ -882,7 +896,8 @@ template <int CP3>
 __global__ __launch_bounds__(64 * TRAIN_WAVES) void bpc_train_kernel(const TrainArgs a) {
   extern __shared__ unsigned long long hist[];
   for (int k = threadIdx.x; k < a.lut_len; k += blockDim.x) hist[k] = 0;
-  __syncthreads();
+  __shared__ uint8_t slut[256];
+  const lds_u8 sl = build_sign_lut(slut);  // its barrier also covers the histogram clear
   const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * TRAIN_WAVES + (threadIdx.x >> 6)));
   const int nwaves = (int)gridDim.x * TRAIN_WAVES;
   void* scr = reinterpret_cast<char*>(a.scratch) + (size_t)wave * scratch_bytes_per_wave(TRAIN_MAXP);
@@ -901,6 +916,7 @@ __global__ __launch_bounds__(64 * TRAIN_WAVES) void bpc_train_kernel(const Train
   d.layout = a.layout;
   d.fill = 0;
   d.in16 = 0;
+  d.slut = sl;
   const int npairs = (a.ncb + 1) / 2;
   for (int pair = wave; pair < npairs; pair += nwaves) {
     const int cb = 2 * pair + ((threadIdx.x & 63) >> 5);
