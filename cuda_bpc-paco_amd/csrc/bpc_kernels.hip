@@ -62,6 +62,9 @@ __device__ __forceinline__ uint8_t sign_lut_entry(u32 k) {
   return (uint8_t)(((s4 >> 1) << 3) | ((s4 & 1u) << 5));
 }
 typedef __attribute__((address_space(3))) const uint8_t* lds_u8;
+typedef __attribute__((address_space(3))) uint8_t* lds_w8;
+typedef __attribute__((address_space(3))) uint32_t* lds_w32;
+constexpr int PROW_BYTES = 20;  // a lane's row: 9 significance, 9 cleanup-pass probabilities
 __device__ __forceinline__ lds_u8 build_sign_lut(uint8_t* t) {
   for (u32 k = threadIdx.x; k < 256; k += blockDim.x) t[k] = sign_lut_entry(k);
   __syncthreads();
@@ -69,8 +72,8 @@ __device__ __forceinline__ lds_u8 build_sign_lut(uint8_t* t) {
 }
 
 struct PlaneLut {  // probabilities of one bitplane, 7 bits each
-  u64 sig;         // 9 significance contexts, entry k at bits 7k
-  u64 cpsig;       // cleanup-pass significance contexts
+  u64 sig;         // 9 significance contexts, entry k at bits 7k (per-image kernel; the
+  u64 cpsig;       // batched kernel keeps them in the lane's LDS row, load_plane_lut)
   u32 sign;        // 4 sign contexts, entry k at bits 8k
   u32 cpsign;
   u32 ref;         // refinement probability
@@ -125,6 +128,7 @@ struct Geo {
   u32 edge_odd;    // 7, or 0 for the last lane of a half (no right neighbour)
   unsigned long long* hist;  // symbol-count mode: per LUT entry, (symbols << 32) + zeros, in LDS
   lds_u8 slut;               // sign contexts (build_sign_lut)
+  lds_u8 prow;               // this lane's significance probabilities of the plane: [0, 9) SPP, [9, 18) CP
   // branch-free coder: the wave's codestream window (both halves' regions) and the
   // byte offset of this lane's region in it
   __amdgpu_buffer_rsrc_t cs;
@@ -300,12 +304,12 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
     } else if (BF) {
       enc_sel(s, at, pbit, (u32)((CLEAN ? pl.cpsig : pl.sig) >> (7u * ctx)) & 127u, g);
     } else if (at) {
-      enc_symbol(s, pbit, (u32)((CLEAN ? pl.cpsig : pl.sig) >> (7u * ctx)) & 127u, g, dst);
+      enc_symbol(s, pbit, (u32)g.prow[(CLEAN ? 9u : 0u) + ctx], g, dst);
     }
   } else if (BF) {
     sym = dec_sel(s, at, (u32)((CLEAN ? pl.cpsig : pl.sig) >> (7u * ctx)) & 127u);
   } else if (at) {
-    sym = dec_symbol(s, (u32)((CLEAN ? pl.cpsig : pl.sig) >> (7u * ctx)) & 127u);
+    sym = dec_symbol(s, (u32)g.prow[(CLEAN ? 9u : 0u) + ctx]);
   }
   // a 1 makes the coefficient significant (and, in CP, refined: BPC_PaCo.cu:534;
   // in the decoder it is magnitude bit i of this plane), without a branch
@@ -476,24 +480,48 @@ __device__ __forceinline__ u32 lut_at(cgi32 lut, int idx, bool& bad) {
   return (u32)v & 127u;
 }
 
-template <int CP3>
+// The plane's probabilities of the lane's LUT group (lanes of one codeblock can
+// sit in different subbands).  ROWS = 1 (batched kernel): the significance
+// probabilities go to the lane's LDS row as bytes (entry k of SPP at byte k, of CP
+// at byte 9 + k), where a scan step reads one with a byte load; with 8 waves per
+// SIMD that latency is hidden and the three VALU of a register lookup are saved.
+// ROWS = 0 (per-image kernel, latency-bound): they stay packed in registers.
+template <int CP3, int ROWS>
 __device__ __forceinline__ PlaneLut load_plane_lut(cgi32 lut, int ref0, int sig0, int sign0, int cpoff,
-                                                   int bp, bool& bad) {
+                                                   int bp, bool& bad, lds_w8 row) {
   PlaneLut pl;
   pl.sig = 0;
   pl.cpsig = 0;
   pl.sign = 0;
   pl.cpsign = 0;
+  u32 w[5] = {0, 0, 0, 0, 0};
 #pragma unroll
-  for (int k = 0; k < N_SIG; ++k) pl.sig |= (u64)lut_at(lut, sig0 + bp * N_SIG + k, bad) << (7 * k);
+  for (int k = 0; k < N_SIG; ++k) {
+    const u32 v = lut_at(lut, sig0 + bp * N_SIG + k, bad);
+    if (ROWS)
+      w[k >> 2] |= v << (8 * (k & 3));
+    else
+      pl.sig |= (u64)v << (7 * k);
+  }
 #pragma unroll
   for (int k = 0; k < N_SIGN; ++k) pl.sign |= lut_at(lut, sign0 + bp * N_SIGN + k, bad) << (8 * k);
   pl.ref = lut_at(lut, ref0 + bp, bad);
   if (CP3) {
 #pragma unroll
-    for (int k = 0; k < N_SIG; ++k) pl.cpsig |= (u64)lut_at(lut, cpoff + sig0 + bp * N_SIG + k, bad) << (7 * k);
+    for (int k = 0; k < N_SIG; ++k) {
+      const u32 v = lut_at(lut, cpoff + sig0 + bp * N_SIG + k, bad);
+      if (ROWS)
+        w[(N_SIG + k) >> 2] |= v << (8 * ((N_SIG + k) & 3));
+      else
+        pl.cpsig |= (u64)v << (7 * k);
+    }
 #pragma unroll
     for (int k = 0; k < N_SIGN; ++k) pl.cpsign |= lut_at(lut, cpoff + sign0 + bp * N_SIGN + k, bad) << (8 * k);
+  }
+  if (ROWS) {
+    const lds_w32 r = (lds_w32)row;
+#pragma unroll
+    for (int q = 0; q < (CP3 ? 5 : 3); ++q) r[q] = w[q];
   }
   return pl;
 }
@@ -509,6 +537,7 @@ struct HalfDesc {
   int in16;  // coefficients as int16 instead of int32
   int cb;    // codeblock of this half in its image, -1 for an empty half
   lds_u8 slut;  // sign contexts in LDS (build_sign_lut)
+  lds_w8 prows; // this wave's LDS rows, PROW_BYTES per lane (unused in symbol-count mode)
 };
 
 // Codes the two codeblocks of one wave (Kernel_BPC_CODER / Kernel_BPC_DECODER
@@ -620,6 +649,8 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
   g.edge_odd = hl == 31 ? 0u : 7u;
   g.hist = hist;
   g.slut = a.slut;
+  const lds_w8 myrow = a.prows + PROW_BYTES * lane;
+  g.prow = (lds_u8)myrow;
   if (BF) {
     // the window spans both halves' regions (two codeblocks of one image)
     const bool v0 = __builtin_amdgcn_readlane((int)valid, 0), v1 = __builtin_amdgcn_readlane((int)valid, 32);
@@ -732,7 +763,7 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
       pl.icp = cpoff;
       pl.tal = bp < a.nbp;
     } else if (act) {
-      pl = load_plane_lut<CP3>(a.lut, ref0, sig0, sign0, cpoff, bp, lbad);
+      pl = load_plane_lut<CP3, BF ? 0 : 1>(a.lut, ref0, sig0, sign0, cpoff, bp, lbad, myrow);
     }
     if (lbad) status |= ST_RANGE;
     u64 next[2] = {0, 0};
@@ -828,6 +859,7 @@ __global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
   d.fill = a.fill;
   d.in16 = a.in16;
   d.slut = sl;
+  d.prows = nullptr;  // the branch-free form keeps its probabilities in registers
   const int half = (threadIdx.x & 63) >> 5;
   const int nunits = a.single ? a.ncb : (a.ncb + 1) / 2;
   for (int unit = wave; unit < nunits; unit += nwaves) {
@@ -850,7 +882,9 @@ template <int DEC, int MAXP, int OCC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC ? OCC : 1, OCC ? OCC : 8)))
 void bpc_batch_kernel(const BatchArgs a) {
   __shared__ uint8_t slut[256];
+  __shared__ uint32_t prows[4 * 64 * PROW_BYTES / 4];  // per wave of the workgroup (at most 4)
   const lds_u8 sl = build_sign_lut(slut);
+  const lds_w8 pr = (lds_w8)prows + 64 * PROW_BYTES * (threadIdx.x >> 6);
   const int half = (threadIdx.x & 63) >> 5;
   const size_t wave = (size_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   void* scr = reinterpret_cast<char*>(a.scratch) + wave * (2 * MAXP * 64 * 8);
@@ -879,6 +913,7 @@ void bpc_batch_kernel(const BatchArgs a) {
     d.fill = im.fill;
     d.in16 = 0;
     d.slut = sl;
+    d.prows = pr;
     d.cb = e == QUEUE_EMPTY ? -1 : (int)(e & 0xFFFFFu);
     if (im.cpass3)
       code_pair<DEC, 1, MAXP>(d, a.rows, &a.ctl[CTL_STATUS], scr);
@@ -917,6 +952,7 @@ __global__ __launch_bounds__(64 * TRAIN_WAVES) void bpc_train_kernel(const Train
   d.fill = 0;
   d.in16 = 0;
   d.slut = sl;
+  d.prows = nullptr;  // symbol counting reads no probabilities
   const int npairs = (a.ncb + 1) / 2;
   for (int pair = wave; pair < npairs; pair += nwaves) {
     const int cb = 2 * pair + ((threadIdx.x & 63) >> 5);
