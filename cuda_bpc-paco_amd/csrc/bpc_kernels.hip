@@ -743,7 +743,9 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
 
   // encoder: magnitude plane `top`, and the next one prefetched during each plane
   u64 plane[2] = {0, 0};
-  if (!DEC && top >= 0) scr.plane(top, plane);
+  // (the per-image kernel prefetches each next plane; the batched kernel, at 8 waves
+  // per SIMD, loads each plane when it starts and spends no registers on a prefetch)
+  if (!DEC && BF && top >= 0) scr.plane(top, plane);
 
   // Bitplane driver (Encode / Decode, BPC_PaCo.cu:968-1090).  In 3-pass mode the
   // reference's opening cleanup pass at MSB equals plane MSB with SPP and MRP
@@ -767,7 +769,8 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
     }
     if (lbad) status |= ST_RANGE;
     u64 next[2] = {0, 0};
-    if (!DEC && bp > 0) scr.plane(bp - 1, next);
+    if (!DEC && BF && bp > 0) scr.plane(bp - 1, next);
+    if (!DEC && !BF) scr.plane(bp, plane);
     const bool act_sr = act && !(CP3 && bp == msb);
     if (ballot(act_sr)) {
       sig_pass<DEC, CP3, 0, TRAIN, BF>(s, act_sr, plane, pl, g, a.src, a.dst);
@@ -775,8 +778,10 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
     }
     if (CP3) sig_pass<DEC, CP3, 1, TRAIN, BF>(s, act, plane, pl, g, a.src, a.dst);
     if (!DEC) {
-      plane[0] = next[0];
-      plane[1] = next[1];
+      if (BF) {
+        plane[0] = next[0];
+        plane[1] = next[1];
+      }
     } else {  // decoded magnitude bits of plane bp -> the wave's scratch
       scr.store(bp, (u32)s.cur[0], (u32)s.cur[1]);
       scr.store(MAXP + bp, (u32)(s.cur[0] >> 32), (u32)(s.cur[1] >> 32));
