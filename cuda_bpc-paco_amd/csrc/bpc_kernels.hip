@@ -94,6 +94,22 @@ struct LaneState {
 // Lane mask of a predicate, taken straight from the compare (no 0/1 round trip).
 __device__ __forceinline__ u64 ballot(bool b) { return __builtin_amdgcn_ballot_w64(b); }
 
+// Lane masks of a compare, written out so that the mask stays in an SGPR pair and
+// combines with other masks on the scalar unit (a ballot of a combined predicate
+// would otherwise be rebuilt through a 0/1 VGPR).  Volatile: the compare must run
+// with the whole wave active, where the caller put it.
+__device__ __forceinline__ u64 lanes_eq0(u32 v) {
+  u64 r;
+  asm volatile("v_cmp_eq_u32_e64 %0, 0, %1" : "=s"(r) : "v"(v));
+  return r;
+}
+__device__ __forceinline__ u64 lanes_ne0(u32 v) {
+  u64 r;
+  asm volatile("v_cmp_ne_u32_e64 %0, 0, %1" : "=s"(r) : "v"(v));
+  return r;
+}
+__device__ __forceinline__ bool lane_in(u64 m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
+
 // Hides a value from the optimizer so that a later compare on it stays a single
 // v_cmp whose lane mask feeds a ballot directly (instead of being folded into a
 // combination of masks that must be rebuilt through a 0/1 VGPR).
@@ -147,10 +163,13 @@ __device__ __forceinline__ void tally(unsigned long long* hist, int idx, u32 sym
 // order after the half's counter; the counter advances by their number.  The
 // counters stay wave-uniform (SGPRs).
 template <int DEC, int MAXP>
-__device__ __forceinline__ void reserve(LaneState<MAXP>& s, bool at, const Geo& g, cgi32 src) {
-  // 0 exactly for the lanes that reserve: one compare feeds the ballot directly
-  const u32 szr = opaque(at ? s.Sz : 1u);
-  const u64 m = ballot(szr == 0);
+__device__ __forceinline__ void reserve(LaneState<MAXP>& s, bool at, u64 atm, const Geo& g, cgi32 src) {
+  // the call site's lanes whose interval is exhausted.  The encoder has the site's
+  // lanes as a wave mask and combines it with one compare on the scalar unit; the
+  // decoder has them as a predicate and feeds one compare (0 exactly for the lanes
+  // that reserve) to the ballot.
+  const u32 szr = DEC ? opaque(at ? s.Sz : 1u) : 1u;
+  const u64 m = DEC ? ballot(szr == 0) : lanes_eq0(s.Sz) & atm;
   if (m) {
     const u32 lo = (u32)m, hi = (u32)(m >> 32);
     const u32 nlo = __builtin_popcount(lo);
@@ -158,7 +177,7 @@ __device__ __forceinline__ void reserve(LaneState<MAXP>& s, bool at, const Geo& 
     // lane too; its base is lowered by that count
     const u32 base = g.half ? s.cnt1 - nlo : s.cnt0;
     const u32 slot = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, base));
-    if (szr == 0) {
+    if (DEC ? szr == 0 : lane_in(m)) {
       // A slot past the region (overflow, reported from the final counter) is
       // redirected to the region's last word so that no access leaves the region.
       s.slot = min(slot, (u32)g.stride - 1u);
@@ -169,6 +188,22 @@ __device__ __forceinline__ void reserve(LaneState<MAXP>& s, bool at, const Geo& 
     s.cnt0 = __builtin_amdgcn_readfirstlane(s.cnt0 + nlo);
     s.cnt1 = __builtin_amdgcn_readfirstlane(s.cnt1 + __builtin_popcount(hi));
   }
+}
+
+// (a << SH) | c as one v_lshl_or_b32 (the compiler would otherwise form two
+// shifts and a three-way OR)
+template <int SH>
+__device__ __forceinline__ u32 lshl_or(u32 a, u32 c) {
+  u32 r;
+  asm("v_lshl_or_b32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "i"(SH), "v"(c));
+  return r;
+}
+
+// (a & b) | c as one v_and_or_b32
+__device__ __forceinline__ u32 and_or(u32 a, u32 b, u32 c) {
+  u32 r;
+  asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+  return r;
 }
 
 // a * b + c with 24-bit a, b (one v_mad_u32_u24; written out because the compiler
@@ -262,7 +297,7 @@ __device__ __forceinline__ u32 window(u64 x, int i) {
 // BPC_PaCo.cu:412-507) or of the cleanup pass (CLEAN = 1, CP_Encoder/CP_Decoder,
 // BPC_PaCo.cu:510-589) at row i (in word H, window form W) of column ODD.
 template <int ODD, int H, int W, int DEC, int CP3, int CLEAN, int TRAIN, int BF, int MAXP>
-__device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u64 plane, const PlaneLut& pl,
+__device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u64 actm, u64 plane, const PlaneLut& pl,
                                          const Geo& g, cgi32 src, gi32 dst) {
   constexpr int c = ODD, o = 1 - ODD;
   const int ib = i - 32 * H;
@@ -276,29 +311,47 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
   const u32 nb = (ODD ? dpp_from_right(wo) : dpp_from_left(wo)) & (ODD ? g.edge_odd : g.edge_even);
   // context = significant neighbours; wc includes the coefficient itself, which is
   // not significant wherever a context is used
-  const u32 nbr = wo | (wc << 3) | (nb << 6);
+  const u32 nbr = lshl_or<6>(nb, lshl_or<3>(wc, wo));
   const u32 ctx = __builtin_popcount(nbr);
 
+  // the lanes that code at this step: the decoder forms the predicate, the encoder
+  // the wave mask (compares into SGPR pairs, combined on the scalar unit, which
+  // leaves fewer VALU instructions on the encoder's scan step)
+  u64 atm = 0;
   bool at;
-  if (!CLEAN) {
-    const bool cand = act && !(wc & 2u);
-    if (CP3) {
-      const bool gate = ctx != 0;  // BPC_PaCo.cu:424
-      set_bit<H>(s.C[c], (cand && !gate ? 1u : 0u) << ib);  // BPC_PaCo.cu:424, without a branch
-      at = cand && gate;
+  if (DEC) {
+    if (!CLEAN) {
+      const bool cand = act && !(wc & 2u);
+      if (CP3) {
+        const bool gate = ctx != 0;  // BPC_PaCo.cu:424
+        set_bit<H>(s.C[c], (cand && !gate ? 1u : 0u) << ib);  // BPC_PaCo.cu:424, without a branch
+        at = cand && gate;
+      } else {
+        at = cand;
+      }
     } else {
-      at = cand;
+      at = act && ((word<H>(s.C[c]) >> ib) & 1u);  // BPC_PaCo.cu:515
+      clear_bit<H>(s.C[c], (at ? 1u : 0u) << ib);   // BPC_PaCo.cu:528, without a branch
+    }
+  } else if (!CLEAN) {
+    const u64 candm = actm & lanes_eq0(wc & 2u);
+    if (CP3) {
+      const u64 gatem = lanes_ne0(nbr);  // context != 0, BPC_PaCo.cu:424
+      set_bit<H>(s.C[c], (lane_in(candm & ~gatem) ? 1u : 0u) << ib);  // BPC_PaCo.cu:424, without a branch
+      atm = candm & gatem;
+    } else {
+      atm = candm;
     }
   } else {
-    at = act && ((word<H>(s.C[c]) >> ib) & 1u);  // BPC_PaCo.cu:515
-    clear_bit<H>(s.C[c], (at ? 1u : 0u) << ib);   // BPC_PaCo.cu:528, without a branch
+    atm = actm & lanes_ne0(word<H>(s.C[c]) & bit);  // BPC_PaCo.cu:515
+    clear_bit<H>(s.C[c], (lane_in(atm) ? 1u : 0u) << ib);  // BPC_PaCo.cu:528, without a branch
   }
+  if (!DEC) at = lane_in(atm);
   // site 1: significance symbol
-  if (!TRAIN) reserve<DEC>(s, at, g, src);
-  u32 sym = 0;
+  if (!TRAIN) reserve<DEC>(s, at, atm, g, src);
+  u32 sym = 0;  // decoder: the decoded symbol
   if (!DEC) {
     const u32 pbit = (word<H>(plane) >> ib) & 1u;  // the symbol, known before coding
-    sym = at ? pbit : 0u;
     if (TRAIN) {
       if (at && pl.tal) tally(g.hist, (CLEAN ? pl.isig + pl.icp : pl.isig) + (int)ctx, pbit);
     } else if (BF) {
@@ -312,25 +365,31 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
     sym = dec_symbol(s, (u32)g.prow[(CLEAN ? 9u : 0u) + ctx]);
   }
   // a 1 makes the coefficient significant (and, in CP, refined: BPC_PaCo.cu:534;
-  // in the decoder it is magnitude bit i of this plane), without a branch
-  set_bit<H>(s.S[c], sym << ib);
-  if (CLEAN) set_bit<H>(s.R[c], sym << ib);
-  if (DEC) set_bit<H>(s.cur[c], sym << ib);
-  sym = opaque(sym);
+  // in the decoder it is magnitude bit i of this plane), without a branch.  The
+  // encoder forms the symbol at its row bit straight from the plane word (one AND
+  // and one select).  The branch-free decoder's symbol is a compare result whose
+  // ballot is free; the branchy decoder's is hidden from the optimizer so that its
+  // ballot stays one compare.
+  const u32 symb = DEC ? sym << ib : (at ? (word<H>(plane) & bit) : 0u);  // the symbol at its row bit
+  set_bit<H>(s.S[c], symb);
+  if (CLEAN) set_bit<H>(s.R[c], symb);
+  if (DEC) set_bit<H>(s.cur[c], symb);
+  const u32 coded1 = DEC ? (BF ? sym : opaque(sym)) : symb;
+  const u64 nsm = DEC ? ballot(coded1 != 0) : lanes_ne0(symb);
   // site 2: sign symbol of the coefficients that became significant
-  if (ballot(sym != 0)) {
-    const bool nsig = sym != 0;
+  if (nsm) {
+    const bool nsig = DEC ? coded1 != 0 : lane_in(nsm);
     // neighbour lane's sign bit at row i, moved like its significance window
     const u32 gs = (word<H>(s.G[o]) >> ib) & 1u;
     const u32 gn = ODD ? dpp_from_right(gs) : dpp_from_left(gs);
     // the 8 neighbour bits (build_sign_lut): significance bits straight from the
     // context word, sign bits from the own column's sign window and the two row-i
     // sign bits; the context and flip come from one LDS byte
-    const u32 idx = (nbr & 0xAAu) | ((window<W>(s.G[c], i) << 2) & 0x14u) | gs | (gn << 6);
+    const u32 idx = lshl_or<6>(gn, lshl_or<2>(window<W>(s.G[c], i) & 5u, and_or(nbr, 0xAAu, gs)));
     const u32 e = g.slut[idx];
     const u32 ssh = e & 24u;   // 8 x sign context: its probability's bit offset
     const u32 flip = e >> 5;   // predicted sign
-    if (!TRAIN) reserve<DEC>(s, nsig, g, src);
+    if (!TRAIN) reserve<DEC>(s, nsig, nsm, g, src);
     if (BF && !TRAIN) {
       const u32 prob = ((CLEAN ? pl.cpsign : pl.sign) >> ssh) & 127u;
       const u32 own = (word<H>(s.G[c]) >> ib) & 1u;
@@ -364,12 +423,13 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
 // step's own coefficient, whose flag no later step of the pass reads, so the pass
 // applies it to all coefficients at its end (ref_pass).
 template <int ODD, int H, int DEC, int TRAIN, int BF, int MAXP>
-__device__ __forceinline__ void ref_step(LaneState<MAXP>& s, int i, bool act, u64 plane, const PlaneLut& pl,
+__device__ __forceinline__ void ref_step(LaneState<MAXP>& s, int i, bool act, u64 actm, u64 plane, const PlaneLut& pl,
                                          const Geo& g, cgi32 src, gi32 dst) {
   constexpr int c = ODD;
   const int ib = i - 32 * H;
-  const bool at = act && ((word<H>(s.R[c]) >> ib) & 1u);
-  if (!TRAIN) reserve<DEC>(s, at, g, src);
+  const u64 atm = DEC ? 0 : actm & lanes_ne0(word<H>(s.R[c]) & (1u << ib));
+  const bool at = DEC ? act && ((word<H>(s.R[c]) >> ib) & 1u) : lane_in(atm);
+  if (!TRAIN) reserve<DEC>(s, at, atm, g, src);
   if (BF && !TRAIN) {
     if (!DEC)
       enc_sel(s, at, (word<H>(plane) >> ib) & 1u, pl.ref, g);
@@ -425,48 +485,50 @@ template <int DEC, int CP3, int CLEAN, int TRAIN, int BF, int MAXP>
 __device__ __forceinline__ void sig_pass(LaneState<MAXP>& s, bool act, const u64* plane, const PlaneLut& pl,
                                          const Geo& g, cgi32 src, gi32 dst) {
   const u64 rowmask = g.rows >= 64 ? ~0ull : ((1ull << g.rows) - 1ull);
+  const u64 actm = DEC ? 0 : ballot(act);  // the encoder's steps take the active lanes as a mask
   const u64 mE = wave_or(act ? (CLEAN ? s.C[0] : (~s.S[0] & rowmask)) : 0ull);
   const u64 mO = wave_or(act ? (CLEAN ? s.C[1] : (~s.S[1] & rowmask)) : 0ull);
   const u32 eL = (u32)mE, oL = (u32)mO, eH = (u32)(mE >> 32), oH = (u32)(mO >> 32);
   if ((eL | oL) & 1u) {
-    if (eL & 1u) sig_step<0, 0, 0, DEC, CP3, CLEAN, TRAIN, BF>(s, 0, act, plane[0], pl, g, src, dst);
-    if (oL & 1u) sig_step<1, 0, 0, DEC, CP3, CLEAN, TRAIN, BF>(s, 0, act, plane[1], pl, g, src, dst);
+    if (eL & 1u) sig_step<0, 0, 0, DEC, CP3, CLEAN, TRAIN, BF>(s, 0, act, actm, plane[0], pl, g, src, dst);
+    if (oL & 1u) sig_step<1, 0, 0, DEC, CP3, CLEAN, TRAIN, BF>(s, 0, act, actm, plane[1], pl, g, src, dst);
   }
   for (u32 r = (eL | oL) & 0x7FFFFFFEu; r; r &= r - 1u) {
     const int i = __builtin_ctz(r);
-    if ((eL >> i) & 1u) sig_step<0, 0, 3, DEC, CP3, CLEAN, TRAIN, BF>(s, i, act, plane[0], pl, g, src, dst);
-    if ((oL >> i) & 1u) sig_step<1, 0, 3, DEC, CP3, CLEAN, TRAIN, BF>(s, i, act, plane[1], pl, g, src, dst);
+    if ((eL >> i) & 1u) sig_step<0, 0, 3, DEC, CP3, CLEAN, TRAIN, BF>(s, i, act, actm, plane[0], pl, g, src, dst);
+    if ((oL >> i) & 1u) sig_step<1, 0, 3, DEC, CP3, CLEAN, TRAIN, BF>(s, i, act, actm, plane[1], pl, g, src, dst);
   }
   if ((eL | oL) >> 31) {  // row 31: its window reaches row 32 in the upper word
-    if (eL >> 31) sig_step<0, 0, 1, DEC, CP3, CLEAN, TRAIN, BF>(s, 31, act, plane[0], pl, g, src, dst);
-    if (oL >> 31) sig_step<1, 0, 1, DEC, CP3, CLEAN, TRAIN, BF>(s, 31, act, plane[1], pl, g, src, dst);
+    if (eL >> 31) sig_step<0, 0, 1, DEC, CP3, CLEAN, TRAIN, BF>(s, 31, act, actm, plane[0], pl, g, src, dst);
+    if (oL >> 31) sig_step<1, 0, 1, DEC, CP3, CLEAN, TRAIN, BF>(s, 31, act, actm, plane[1], pl, g, src, dst);
   }
   if ((eH | oH) & 1u) {
-    if (eH & 1u) sig_step<0, 1, 1, DEC, CP3, CLEAN, TRAIN, BF>(s, 32, act, plane[0], pl, g, src, dst);
-    if (oH & 1u) sig_step<1, 1, 1, DEC, CP3, CLEAN, TRAIN, BF>(s, 32, act, plane[1], pl, g, src, dst);
+    if (eH & 1u) sig_step<0, 1, 1, DEC, CP3, CLEAN, TRAIN, BF>(s, 32, act, actm, plane[0], pl, g, src, dst);
+    if (oH & 1u) sig_step<1, 1, 1, DEC, CP3, CLEAN, TRAIN, BF>(s, 32, act, actm, plane[1], pl, g, src, dst);
   }
   for (u32 r = (eH | oH) & ~1u; r; r &= r - 1u) {
     const int i = 32 + __builtin_ctz(r);
-    if ((eH >> (i - 32)) & 1u) sig_step<0, 1, 2, DEC, CP3, CLEAN, TRAIN, BF>(s, i, act, plane[0], pl, g, src, dst);
-    if ((oH >> (i - 32)) & 1u) sig_step<1, 1, 2, DEC, CP3, CLEAN, TRAIN, BF>(s, i, act, plane[1], pl, g, src, dst);
+    if ((eH >> (i - 32)) & 1u) sig_step<0, 1, 2, DEC, CP3, CLEAN, TRAIN, BF>(s, i, act, actm, plane[0], pl, g, src, dst);
+    if ((oH >> (i - 32)) & 1u) sig_step<1, 1, 2, DEC, CP3, CLEAN, TRAIN, BF>(s, i, act, actm, plane[1], pl, g, src, dst);
   }
 }
 
 template <int DEC, int TRAIN, int BF, int MAXP>
 __device__ __forceinline__ void ref_pass(LaneState<MAXP>& s, bool act, const u64* plane, const PlaneLut& pl,
                                          const Geo& g, cgi32 src, gi32 dst) {
+  const u64 actm = DEC ? 0 : ballot(act);  // the encoder's steps take the active lanes as a mask
   const u64 mE = wave_or(act ? s.R[0] : 0ull);  // rows with a refinement symbol in some lane
   const u64 mO = wave_or(act ? s.R[1] : 0ull);
   const u32 eL = (u32)mE, oL = (u32)mO, eH = (u32)(mE >> 32), oH = (u32)(mO >> 32);
   for (u32 r = eL | oL; r; r &= r - 1u) {
     const int i = __builtin_ctz(r);
-    if ((eL >> i) & 1u) ref_step<0, 0, DEC, TRAIN, BF>(s, i, act, plane[0], pl, g, src, dst);
-    if ((oL >> i) & 1u) ref_step<1, 0, DEC, TRAIN, BF>(s, i, act, plane[1], pl, g, src, dst);
+    if ((eL >> i) & 1u) ref_step<0, 0, DEC, TRAIN, BF>(s, i, act, actm, plane[0], pl, g, src, dst);
+    if ((oL >> i) & 1u) ref_step<1, 0, DEC, TRAIN, BF>(s, i, act, actm, plane[1], pl, g, src, dst);
   }
   for (u32 r = eH | oH; r; r &= r - 1u) {
     const int j = __builtin_ctz(r);
-    if ((eH >> j) & 1u) ref_step<0, 1, DEC, TRAIN, BF>(s, 32 + j, act, plane[0], pl, g, src, dst);
-    if ((oH >> j) & 1u) ref_step<1, 1, DEC, TRAIN, BF>(s, 32 + j, act, plane[1], pl, g, src, dst);
+    if ((eH >> j) & 1u) ref_step<0, 1, DEC, TRAIN, BF>(s, 32 + j, act, actm, plane[0], pl, g, src, dst);
+    if ((oH >> j) & 1u) ref_step<1, 1, DEC, TRAIN, BF>(s, 32 + j, act, actm, plane[1], pl, g, src, dst);
   }
   if (act) {  // every significant coefficient refines from the next plane on (BPC_PaCo.cu:604)
     s.R[0] |= s.S[0];
