@@ -73,7 +73,8 @@ __device__ __forceinline__ lds_u8 build_sign_lut(uint8_t* t) {
 
 struct PlaneLut {  // probabilities of one bitplane, 7 bits each
   u64 sig;         // 9 significance contexts, entry k at bits 7k (per-image kernel; the
-  u64 cpsig;       // batched kernel keeps them in the lane's LDS row, load_plane_lut)
+  u64 cpsig;       // batched kernel keeps them in the lane's LDS row, load_plane_lut);
+                   // 3-pass SPP: entries 1..8 as bytes 0..7 (sig_prob)
   u32 sign;        // 4 sign contexts, entry k at bits 8k
   u32 cpsign;
   u32 ref;         // refinement probability
@@ -109,6 +110,13 @@ __device__ __forceinline__ u64 lanes_ne0(u32 v) {
   return r;
 }
 __device__ __forceinline__ bool lane_in(u64 m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
+// t + 1 in the lanes of m, t elsewhere: one add with the mask as carry-in
+__device__ __forceinline__ u32 add_if(u32 t, u64 m) {
+  u32 r;
+  u64 co;
+  asm("v_addc_co_u32_e64 %0, %1, %2, 0, %3" : "=v"(r), "=s"(co) : "v"(t), "s"(m));
+  return r;
+}
 
 // Hides a value from the optimizer so that a later compare on it stays a single
 // v_cmp whose lane mask feeds a ballot directly (instead of being folded into a
@@ -214,6 +222,17 @@ __device__ __forceinline__ u32 mad24(u32 a, u32 b, u32 c) {
   return r;
 }
 
+// Significance probability of context ctx from the plane's packed row.  The 3-pass
+// SPP codes only contexts 1..8 (a coefficient without a significant neighbour is
+// left to the cleanup pass), which fit one 8-byte row: one byte select (v_perm_b32)
+// instead of a 64-bit shift and a mask.  Lanes with context 0 get a value that no
+// kept result depends on.
+template <int CLEAN, int CP3>
+__device__ __forceinline__ u32 sig_prob(const PlaneLut& pl, u32 ctx) {
+  if (!CLEAN && CP3) return __builtin_amdgcn_perm((u32)(pl.sig >> 32), (u32)pl.sig, ctx + 0x0C0C0BFFu);
+  return (u32)((CLEAN ? pl.cpsig : pl.sig) >> (7u * ctx)) & 127u;
+}
+
 // Encoder coder step (BPC_PaCo.cu:281-294).
 template <int MAXP>
 __device__ __forceinline__ void enc_symbol(LaneState<MAXP>& s, u32 sym, u32 prob, const Geo& g, gi32 dst) {
@@ -237,6 +256,16 @@ __device__ __forceinline__ void enc_sel(LaneState<MAXP>& s, bool at, u32 sym, u3
   const u32 nl = mad24(aux, sym, s.L);
   s.Sz = at ? nsz : s.Sz;
   s.L = at ? nl : s.L;
+  const bool close = at && nsz == 0;
+  __builtin_amdgcn_raw_buffer_store_b32(s.L, g.cs, close ? g.cs_off + s.slot * 4u : CS_DISCARD, 0, 0);
+}
+template <int MAXP>
+__device__ __forceinline__ void enc_sel_m(LaneState<MAXP>& s, bool at, u64 atm, u64 pm, u32 prob, const Geo& g) {
+  const u32 t = __umul24(s.Sz, prob) >> 7;
+  const u32 aux = add_if(t, pm);
+  const u32 nsz = lane_in(pm) ? s.Sz - aux : t;
+  s.L = lane_in(atm & pm) ? s.L + aux : s.L;
+  s.Sz = at ? nsz : s.Sz;
   const bool close = at && nsz == 0;
   __builtin_amdgcn_raw_buffer_store_b32(s.L, g.cs, close ? g.cs_off + s.slot * 4u : CS_DISCARD, 0, 0);
 }
@@ -350,17 +379,21 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
   // site 1: significance symbol
   if (!TRAIN) reserve<DEC>(s, at, atm, g, src);
   u32 sym = 0;  // decoder: the decoded symbol
+  // encoder: the symbol, known before coding, at its row bit (and, for the
+  // branch-free coder, as a lane mask)
+  const u32 pb = DEC || !BF ? 0u : word<H>(plane) & bit;
+  const u64 pm = DEC || !BF ? 0ull : lanes_ne0(pb);
   if (!DEC) {
-    const u32 pbit = (word<H>(plane) >> ib) & 1u;  // the symbol, known before coding
+    const u32 pbit = (word<H>(plane) >> ib) & 1u;
     if (TRAIN) {
       if (at && pl.tal) tally(g.hist, (CLEAN ? pl.isig + pl.icp : pl.isig) + (int)ctx, pbit);
     } else if (BF) {
-      enc_sel(s, at, pbit, (u32)((CLEAN ? pl.cpsig : pl.sig) >> (7u * ctx)) & 127u, g);
+      enc_sel_m(s, at, atm, pm, sig_prob<CLEAN, CP3>(pl, ctx), g);
     } else if (at) {
       enc_symbol(s, pbit, (u32)g.prow[(CLEAN ? 9u : 0u) + ctx], g, dst);
     }
   } else if (BF) {
-    sym = dec_sel(s, at, (u32)((CLEAN ? pl.cpsig : pl.sig) >> (7u * ctx)) & 127u);
+    sym = dec_sel(s, at, sig_prob<CLEAN, CP3>(pl, ctx));
   } else if (at) {
     sym = dec_symbol(s, (u32)g.prow[(CLEAN ? 9u : 0u) + ctx]);
   }
@@ -370,12 +403,12 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
   // and one select).  The branch-free decoder's symbol is a compare result whose
   // ballot is free; the branchy decoder's is hidden from the optimizer so that its
   // ballot stays one compare.
-  const u32 symb = DEC ? sym << ib : (at ? (word<H>(plane) & bit) : 0u);  // the symbol at its row bit
+  const u32 symb = DEC ? sym << ib : (at ? (BF ? pb : word<H>(plane) & bit) : 0u);  // the symbol at its row bit
   set_bit<H>(s.S[c], symb);
   if (CLEAN) set_bit<H>(s.R[c], symb);
   if (DEC) set_bit<H>(s.cur[c], symb);
   const u32 coded1 = DEC ? (BF ? sym : opaque(sym)) : symb;
-  const u64 nsm = DEC ? ballot(coded1 != 0) : lanes_ne0(symb);
+  const u64 nsm = DEC ? ballot(coded1 != 0) : (BF ? atm & pm : lanes_ne0(symb));
   // site 2: sign symbol of the coefficients that became significant
   if (nsm) {
     const bool nsig = DEC ? coded1 != 0 : lane_in(nsm);
@@ -562,6 +595,8 @@ __device__ __forceinline__ PlaneLut load_plane_lut(cgi32 lut, int ref0, int sig0
     const u32 v = lut_at(lut, sig0 + bp * N_SIG + k, bad);
     if (ROWS)
       w[k >> 2] |= v << (8 * (k & 3));
+    else if (CP3)
+      pl.sig |= k ? (u64)v << (8 * (k - 1)) : 0ull;  // context 0 never codes in the 3-pass SPP
     else
       pl.sig |= (u64)v << (7 * k);
   }
