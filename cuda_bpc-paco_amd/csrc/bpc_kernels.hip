@@ -623,8 +623,8 @@ __device__ __forceinline__ PlaneLut load_plane_lut(cgi32 lut, int ref0, int sig0
   return pl;
 }
 
-// The image one half-wave codes: wave-uniform in the per-image kernel,
-// half-uniform in the batched kernel (the halves may belong to different images).
+// The image a wave codes and the codeblock of each half (both halves of a wave
+// always code the same image, in the per-image and in the batched kernel).
 struct HalfDesc {
   cgi32 src;
   gi32 dst;
