@@ -76,10 +76,17 @@ class Image(ctypes.Structure):
     ]
 
 
+MEM_INFO_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t),
+                               ctypes.POINTER(ctypes.c_size_t))
+
+
 class ServerConfig(ctypes.Structure):
+    """bpc_server_config."""
     _fields_ = [("max_inflight", ctypes.c_int32), ("max_batch_images", ctypes.c_int32),
                 ("max_batch_codeblocks", ctypes.c_longlong), ("hbm_budget_bytes", ctypes.c_size_t),
-                ("waves_per_block", ctypes.c_int32), ("vgpr_cap_waves", ctypes.c_int32)]
+                ("hbm_reserve_bytes", ctypes.c_size_t), ("pool_limit_bytes", ctypes.c_size_t),
+                ("waves_per_block", ctypes.c_int32), ("vgpr_cap_waves", ctypes.c_int32),
+                ("mem_info", MEM_INFO_FN), ("mem_info_ctx", ctypes.c_void_p)]
 
 
 class Request(ctypes.Structure):
@@ -172,6 +179,7 @@ def lib():
             "bpc_batch_set_tuning": (i32, [P, i32, i32]),
             "bpc_batch_set_vgpr_cap": (i32, [P, i32]),
             "bpc_batch_run": (i32, [P, i32, ctypes.POINTER(Image), i32, i32, P, P]),
+            "bpc_batch_graph_count": (i32, [P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -521,6 +529,12 @@ class Batch:
         _check(lib().bpc_batch_run(self.ptr, direction, arr, len(images), 1 if use_graph else 0,
                                    d_status.ptr if d_status is not None else None, _sp(stream)))
 
+    def graph_count(self):
+        n = lib().bpc_batch_graph_count(self.ptr)
+        if n < 0:
+            _check(n)
+        return n
+
     def destroy(self):
         if self.ptr is not None:
             lib().bpc_batch_destroy(self.ptr)
@@ -542,16 +556,34 @@ def make_image(src, dst, lut, size_x, size_y, p, ncw=None, fill_unused=1):
 # ---- continuous-batching server ------------------------------------------------------
 
 class Server:
-    """Request server: submit encode/decode requests on device buffers, wait for results."""
+    """Request server: submit encode/decode requests on device buffers, wait for results.
 
-    def __init__(self, max_inflight=2, max_batch_images=64, max_batch_codeblocks=1 << 18, hbm_budget_bytes=32 << 30,
-                 waves_per_block=4, vgpr_cap_waves=8):
+    mem_info: optional callable returning (free_bytes, total_bytes), used by admission
+    instead of the device's figures (to exercise admission under memory pressure)."""
+
+    def __init__(self, max_inflight=2, max_batch_images=64, max_batch_codeblocks=1 << 18, hbm_budget_bytes=0,
+                 hbm_reserve_bytes=None, pool_limit_bytes=None, waves_per_block=4, vgpr_cap_waves=8, mem_info=None):
         cfg = ServerConfig()
         lib().bpc_server_default_config(ctypes.byref(cfg))
         cfg.max_inflight, cfg.max_batch_images = max_inflight, max_batch_images
         cfg.max_batch_codeblocks, cfg.hbm_budget_bytes = max_batch_codeblocks, hbm_budget_bytes
+        if hbm_reserve_bytes is not None:
+            cfg.hbm_reserve_bytes = hbm_reserve_bytes
+        if pool_limit_bytes is not None:
+            cfg.pool_limit_bytes = pool_limit_bytes
         cfg.waves_per_block = waves_per_block
         cfg.vgpr_cap_waves = vgpr_cap_waves
+        self._probe = None
+        if mem_info is not None:
+            def probe(ctx, fr, tot):
+                try:
+                    f, t = mem_info()
+                    fr[0], tot[0] = int(f), int(t)
+                    return 0
+                except Exception:
+                    return 1
+            self._probe = MEM_INFO_FN(probe)  # kept alive as long as the server
+            cfg.mem_info = self._probe
         p = ctypes.c_void_p()
         _check(lib().bpc_server_create(ctypes.byref(p), ctypes.byref(cfg)))
         self.ptr = p

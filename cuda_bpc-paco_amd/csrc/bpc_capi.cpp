@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <list>
 #include <map>
 #include <atomic>
 #include <mutex>
@@ -18,19 +19,13 @@
 #include "../../include/bpc_paco.h"
 #include "bpc_internal.h"
 
+using bpc::check_image;
+using bpc::check_params;
+using bpc::fail;
+using bpc::status_to_code;
+using bpc::stride_of;
+
 namespace {
-
-thread_local std::string g_last_error;
-
-int fail(int code, const char* fmt, ...) {
-  char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof buf, fmt, ap);
-  va_end(ap);
-  g_last_error = buf;
-  return code;
-}
 
 int hip_fail(hipError_t e, const char* what) {
   return fail(BPC_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
@@ -41,42 +36,6 @@ int hip_fail(hipError_t e, const char* what) {
     hipError_t e_ = (expr);                             \
     if (e_ != hipSuccess) return hip_fail(e_, #expr);   \
   } while (0)
-
-std::mutex g_cfg_mu;
-bpc_params g_cfg = {0, 5, 64, 15, 0, BPC_LAYOUT_RASTER};
-
-int stride_of(const bpc_params* p) { return p->cb_stride_words > 0 ? p->cb_stride_words : 64 * p->cblock_length; }
-
-int check_params(const bpc_params* p) {
-  if (!p) return fail(BPC_ERR_ARG, "null params");
-  if (p->cpass3 != 0 && p->cpass3 != 1) return fail(BPC_ERR_ARG, "cpass3 must be 0 or 1");
-  if (p->dwt_levels < 0 || p->dwt_levels > 16) return fail(BPC_ERR_ARG, "dwt_levels out of range");
-  if (p->cblock_length < 2 || p->cblock_length > 64) return fail(BPC_ERR_ARG, "cblock_length must be 2..64");
-  if (p->lut_n_bitplanes < 1 || p->lut_n_bitplanes > 28) return fail(BPC_ERR_ARG, "lut_n_bitplanes must be 1..28");
-  if (p->cb_stride_words < 0 || (p->cb_stride_words > 0 && p->cb_stride_words < 2) ||
-      p->cb_stride_words > (1 << 26))
-    return fail(BPC_ERR_ARG, "cb_stride_words must be 0 or 2..2^26");
-  if (p->layout != BPC_LAYOUT_RASTER && p->layout != BPC_LAYOUT_TILED) return fail(BPC_ERR_ARG, "bad layout");
-  return BPC_OK;
-}
-
-int check_image(int size_x, int size_y, const bpc_params* p) {
-  if (size_x <= 0 || size_y <= 0 || size_x % 64 != 0 || size_y % p->cblock_length != 0)
-    return fail(BPC_ERR_ARG, "image %dx%d is not a whole number of 64x%d codeblocks", size_x, size_y,
-                p->cblock_length);
-  if ((int64_t)size_x * size_y > (int64_t)1 << 31) return fail(BPC_ERR_ARG, "image too large");
-  return BPC_OK;
-}
-
-int status_to_code(uint32_t st) {
-  if (st & bpc::ST_MALFORMED) return fail(BPC_ERR_MALFORMED, "decoder: malformed MSB word in a codeblock region");
-  if (st & bpc::ST_RANGE)
-    return fail(BPC_ERR_RANGE, "coefficient magnitude >= 2^28, LUT entry outside [0,127] or LUT row outside the array");
-  if (st & bpc::ST_OVERFLOW) return fail(BPC_ERR_OVERFLOW, "codestream region capacity exceeded");
-  if (st & bpc::ST_WIDE) return fail(BPC_ERR_WIDE, "codeblock MSB beyond the kernel's bitplane registers");
-  if (st & bpc::ST_NARROW) return fail(BPC_ERR_RANGE, "decoded coefficient outside the int16 range (saturated)");
-  return BPC_OK;
-}
 
 // one library-owned status word per device
 std::mutex g_status_mu;
@@ -218,213 +177,14 @@ int check_device_call(const void* a, const void* b, int size_x, int size_y, cons
   return BPC_OK;
 }
 
-// ---- LUT text format (main_example.cu:219-448) ----------------------------
-
-struct LutRow {
-  int bp;
-  int v[9];
-};
-
-int read_lut_file(const std::string& path, int nvals, std::vector<LutRow>& rows) {
-  FILE* f = fopen(path.c_str(), "rb");
-  if (!f) return fail(BPC_ERR_ARG, "cannot open %s", path.c_str());
-  int lvl, sb, bp;
-  char colon[4];
-  while (fscanf(f, "%d %d %d %3s", &lvl, &sb, &bp, colon) == 4) {
-    LutRow r;
-    r.bp = bp;
-    for (int k = 0; k < nvals; ++k)
-      if (fscanf(f, "%d", &r.v[k]) != 1) {
-        fclose(f);
-        return fail(BPC_ERR_ARG, "malformed row in %s", path.c_str());
-      }
-    rows.push_back(r);
-  }
-  fclose(f);
-  return BPC_OK;
-}
-
-// Region fill with the loader's rule: a row whose bitplane is <= the previous
-// one starts the next group; before moving on, (maxbp - prev)*N entries from
-// i + prev*N + 1 are set to 64 (main_example.cu:292-299 and its siblings).
-int place_region(const std::vector<LutRow>& rows, int N, int maxbp, int base, int32_t* lut, int cap) {
-  int i = base, prev = -1;
-  for (const LutRow& r : rows) {
-    if (r.bp <= prev) {
-      for (int z = 0; z < (maxbp - prev) * N; ++z) {
-        const int idx = i + prev * N + z + 1;
-        if (idx >= 0 && idx < cap) lut[idx] = 64;
-      }
-      i += maxbp * N;
-    }
-    prev = r.bp;
-    for (int k = 0; k < N; ++k) {
-      const int idx = i + r.bp * N + k;
-      if (idx < 0 || idx >= cap) return fail(BPC_ERR_ARG, "LUT file row outside the table");
-      lut[idx] = r.v[k];
-    }
-  }
-  return BPC_OK;
-}
-
-// ---- LUT trainer --------------------------------------------------------
-
-constexpr uint32_t S_BIT = 0x80000000u, C_BIT = 0x40000000u, R_BIT = 0x20000000u;
-
-struct Counts {
-  // [region][group][bitplane][context] -> (n0, n)
-  std::vector<uint32_t> n0, n;
-};
-
-int group_of(int x, int y, int X, int Y, int levels) {
-  for (int aux = 1; aux <= levels; ++aux) {
-    const bool bx = x >= (X >> aux), by = y >= (Y >> aux);
-    if (bx || by) return (aux - 1) * 3 + (bx ? (by ? 2 : 0) : 1);
-  }
-  return levels * 3;
-}
-
-int chi_of(uint32_t w) { return (w >> 31) == 0 ? 0 : ((w & 1) ? -1 : 1); }
-
-int sign_ctx(int h, int v) {
-  if (h == 0) return v == 0 ? 0 : (v > 0 ? 2 : 3);
-  if (h > 0) return v == 0 ? 4 : (v > 0 ? 6 : 0);
-  return v == 0 ? 5 : (v > 0 ? 1 : 7);
-}
-
-// Models one codeblock with the coder's pass and scan order (rows top to bottom,
-// even columns then odd columns of each row) and counts the coded symbols.
-void train_codeblock(const int32_t* img, int X, int Y, int cb, const bpc_params* p, int G, int nbp,
-                     uint32_t* n0, uint32_t* n) {
-  const int rows = p->cblock_length;
-  const int nx = X / 64;
-  const int tx = cb % nx, ty = cb / nx;
-  static thread_local std::vector<uint32_t> wbuf;
-  wbuf.assign((size_t)rows * 64, 0);
-  uint32_t* w = wbuf.data();
-  uint32_t m = 0;
-  for (int r = 0; r < rows; ++r)
-    for (int c = 0; c < 64; ++c) {
-      const int32_t v = img[(int64_t)(ty * rows + r) * X + tx * 64 + c];
-      const uint32_t mag = v < 0 ? 0u - (uint32_t)v : (uint32_t)v;
-      if (mag >= (1u << 28)) return;
-      m |= mag;
-      w[r * 64 + c] = (mag << 1) | (v < 0 ? 1u : 0u) | (p->cpass3 ? C_BIT : 0u);
-    }
-  if (m == 0) return;
-  int msb = 31;
-  while (!(m >> msb)) --msb;
-  int grp[64];
-  for (int c = 0; c < 64; c += 2) grp[c] = grp[c + 1] = group_of(tx * 64 + c, ty * rows, X, Y, p->dwt_levels);
-  // region order in the count arrays: 0 ref(1), 1 sig(9), 2 sign(4), 3 cp_sig(9), 4 cp_sign(4)
-  auto count = [&](int region, int g, int bp, int ctx, uint32_t sym) {
-    if (bp >= nbp) return;
-    const size_t idx = (((size_t)region * G + g) * nbp + bp) * 9 + ctx;
-    n[idx]++;
-    if (!sym) n0[idx]++;
-  };
-  auto at = [&](int r, int c) -> uint32_t {
-    if (r < 0 || r >= rows || c < 0 || c >= 64) return 0u;
-    return w[r * 64 + c];
-  };
-  auto sig_pass = [&](int bp, bool clean) {
-    for (int r = 0; r < rows; ++r)
-      for (int par = 0; par < 2; ++par)
-        for (int c = par; c < 64; c += 2) {
-          uint32_t& x = w[r * 64 + c];
-          const uint32_t nb[8] = {at(r - 1, c - 1), at(r - 1, c), at(r - 1, c + 1), at(r, c - 1),
-                                  at(r, c + 1),     at(r + 1, c - 1), at(r + 1, c), at(r + 1, c + 1)};
-          int ctx = 0;
-          for (int k = 0; k < 8; ++k) ctx += (int)(nb[k] >> 31);
-          if (!clean) {
-            if (x & S_BIT) continue;
-            if (p->cpass3 && ctx == 0) {
-              x |= C_BIT;
-              continue;
-            }
-          } else {
-            if (!(x & C_BIT)) continue;
-            x &= ~C_BIT;
-          }
-          const uint32_t sym = (x >> (bp + 1)) & 1u;
-          count(clean ? 3 : 1, grp[c], bp, ctx, sym);
-          if (sym) {
-            x |= S_BIT;
-            if (clean) x |= R_BIT;
-            const int sc = sign_ctx(chi_of(nb[3]) + chi_of(nb[4]), chi_of(nb[1]) + chi_of(nb[6]));
-            count(clean ? 4 : 2, grp[c], bp, sc >> 1, ((x & 1u) == (uint32_t)(sc & 1)) ? 0u : 1u);
-          }
-        }
-  };
-  auto ref_pass = [&](int bp) {
-    for (int r = 0; r < rows; ++r)
-      for (int par = 0; par < 2; ++par)
-        for (int c = par; c < 64; c += 2) {
-          uint32_t& x = w[r * 64 + c];
-          if (x & R_BIT)
-            count(0, grp[c], bp, 0, (x >> (bp + 1)) & 1u);
-          else if (x & S_BIT)
-            x |= R_BIT;
-        }
-  };
-  int bp = msb;
-  if (p->cpass3) {
-    sig_pass(bp, true);
-    bp--;
-  }
-  for (; bp >= 0; --bp) {
-    sig_pass(bp, false);
-    ref_pass(bp);
-    if (p->cpass3) sig_pass(bp, true);
-  }
-}
-
 }  // namespace
 
 extern "C" {
-
-void bpc_default_params(bpc_params* p) {
-  if (!p) return;
-  p->cpass3 = 0;
-  p->dwt_levels = 5;
-  p->cblock_length = 64;
-  p->lut_n_bitplanes = 15;
-  p->cb_stride_words = 0;
-  p->layout = BPC_LAYOUT_RASTER;
-}
-
-int bpc_lut_len(const bpc_params* p) {
-  if (!p) return 0;
-  const int G = 3 * p->dwt_levels + 1;
-  return G * p->lut_n_bitplanes * (1 + 9 + 4 + (p->cpass3 ? 9 + 4 : 0));
-}
-
-int64_t bpc_stream_words(int size_x, int size_y, const bpc_params* p) {
-  if (!p || p->cblock_length <= 0 || size_x <= 0 || size_y <= 0) return 0;
-  return (int64_t)(size_x / 64) * (size_y / p->cblock_length) * stride_of(p);
-}
-
-const char* bpc_last_error(void) { return g_last_error.c_str(); }
 
 int bpc_device_count(void) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
   return n;
-}
-
-int bpc_set_config(const bpc_params* p) {
-  int st = check_params(p);
-  if (st) return st;
-  std::lock_guard<std::mutex> lk(g_cfg_mu);
-  g_cfg = *p;
-  return BPC_OK;
-}
-
-int bpc_get_config(bpc_params* p) {
-  if (!p) return fail(BPC_ERR_ARG, "null params");
-  std::lock_guard<std::mutex> lk(g_cfg_mu);
-  *p = g_cfg;
-  return BPC_OK;
 }
 
 int bpc_encode_device(const int32_t* d_coeffs, int32_t* d_stream, int size_x, int size_y, const int32_t* d_lut,
@@ -548,40 +308,6 @@ int bpc_run(int direction, int32_t* hdata, int size_x, int size_y, const int32_t
   return bpc_run_ex(direction, hdata, size_x, size_y, d_lut, bpc_lut_len(&p), &p);
 }
 
-int bpc_lut_parse_text(const char* folder, int cpass3, int dwt_levels, int32_t* host_lut, int cap, int* lut_len,
-                       int* nbp) {
-  if (!folder || !host_lut || cap <= 0) return fail(BPC_ERR_ARG, "bad arguments");
-  if (cpass3 != 0 && cpass3 != 1) return fail(BPC_ERR_ARG, "cpass3 must be 0 or 1");
-  std::string dir(folder);
-  if (!dir.empty() && dir.back() != '/') dir += '/';
-  const char* names[5] = {"ref.txt", "sig.txt", "sign.txt", "cp_sig.txt", "cp_sign.txt"};
-  const int nv[5] = {1, 9, 4, 9, 4};
-  std::vector<LutRow> rows[5];
-  const int nfiles = cpass3 ? 5 : 3;
-  for (int k = 0; k < nfiles; ++k) {
-    int st = read_lut_file(dir + names[k], nv[k], rows[k]);
-    if (st) return st;
-  }
-  int maxbp = -1;  // taken over ref, sig and sign (main_example.cu:231-264)
-  for (int k = 0; k < 3; ++k)
-    for (const LutRow& r : rows[k]) maxbp = std::max(maxbp, r.bp);
-  maxbp++;
-  if (maxbp < 1) return fail(BPC_ERR_ARG, "empty LUT files");
-  const int G = 3 * dwt_levels + 1;
-  const int refN = G * maxbp, sigN = G * maxbp * 9, signN = G * maxbp * 4;
-  const int total = refN + sigN + signN + (cpass3 ? sigN + signN : 0);
-  if (total > cap) return fail(BPC_ERR_ARG, "LUT needs %d ints, capacity %d", total, cap);
-  for (int k = 0; k < cap; ++k) host_lut[k] = 64;
-  const int bases[5] = {0, refN, refN + sigN, refN + sigN + signN, refN + 2 * sigN + signN};
-  for (int k = 0; k < nfiles; ++k) {
-    int st = place_region(rows[k], nv[k], maxbp, bases[k], host_lut, cap);
-    if (st) return st;
-  }
-  if (lut_len) *lut_len = total;
-  if (nbp) *nbp = maxbp;
-  return BPC_OK;
-}
-
 int bpc_lut_upload(const int32_t* host_lut, int lut_len, int32_t** d_lut) {
   if (!host_lut || lut_len <= 0 || !d_lut) return fail(BPC_ERR_ARG, "bad arguments");
   for (int k = 0; k < lut_len; ++k)
@@ -661,61 +387,6 @@ int bpc_lut_free(int32_t* d_lut) {
   return BPC_OK;
 }
 
-int bpc_lut_train(const int32_t* img, int size_x, int size_y, const bpc_params* p, int32_t* host_lut, int lut_len,
-                  int nthreads) {
-  int st = check_params(p);
-  if (st) return st;
-  st = check_image(size_x, size_y, p);
-  if (st) return st;
-  if (!img || !host_lut) return fail(BPC_ERR_ARG, "null buffer");
-  if (p->layout != BPC_LAYOUT_RASTER) return fail(BPC_ERR_ARG, "training image must be raster");
-  if (lut_len < bpc_lut_len(p)) return fail(BPC_ERR_ARG, "lut_len too small");
-  const int G = 3 * p->dwt_levels + 1;
-  const int nbp = p->lut_n_bitplanes;
-  const size_t ncnt = (size_t)5 * G * nbp * 9;
-  const int ncb = (size_x / 64) * (size_y / p->cblock_length);
-  std::vector<uint32_t> n0(ncnt, 0), n(ncnt, 0);
-  int nt = nthreads > 0 ? nthreads : (int)std::thread::hardware_concurrency();
-  nt = std::max(1, std::min(nt, std::max(1, ncb)));
-  std::atomic<int> next(0);
-  std::mutex mu;
-  std::vector<std::thread> pool;
-  for (int t = 0; t < nt; ++t)
-    pool.emplace_back([&]() {
-      std::vector<uint32_t> l0(ncnt, 0), ln(ncnt, 0);
-      for (;;) {
-        const int cb0 = next.fetch_add(8);
-        if (cb0 >= ncb) break;
-        for (int cb = cb0; cb < std::min(ncb, cb0 + 8); ++cb)
-          train_codeblock(img, size_x, size_y, cb, p, G, nbp, l0.data(), ln.data());
-      }
-      std::lock_guard<std::mutex> lk(mu);
-      for (size_t k = 0; k < ncnt; ++k) {
-        n0[k] += l0[k];
-        n[k] += ln[k];
-      }
-    });
-  for (auto& th : pool) th.join();
-  // layout: ref | sig | sign | cp_sig | cp_sign (BPC_PaCo.cu:228-250, :988-989)
-  const int nctx[5] = {1, 9, 4, 9, 4};
-  int base = 0;
-  for (int region = 0; region < (p->cpass3 ? 5 : 3); ++region) {
-    for (int g = 0; g < G; ++g)
-      for (int bp = 0; bp < nbp; ++bp)
-        for (int ctx = 0; ctx < nctx[region]; ++ctx) {
-          const size_t idx = (((size_t)region * G + g) * nbp + bp) * 9 + ctx;
-          int v = 64;
-          if (n[idx]) {
-            v = (int)(((uint64_t)n0[idx] * 128 + n[idx] / 2) / n[idx]);
-            v = std::min(127, std::max(1, v));
-          }
-          host_lut[base + (g * nbp + bp) * nctx[region] + ctx] = v;
-        }
-    base += G * nbp * nctx[region];
-  }
-  return BPC_OK;
-}
-
 int bpc_dev_malloc(void** ptr, size_t bytes) {
   if (!ptr) return fail(BPC_ERR_ARG, "null pointer");
   HIP_TRY(hipMalloc(ptr, bytes));
@@ -790,20 +461,22 @@ int bpc_mem_info(size_t* free_bytes, size_t* total_bytes) {
 
 // ---- batched codec ---------------------------------------------------------
 
+// Staging of one run, uploaded in one copy: the run words, the offset tables and
+// the image descriptors, each at a fixed place for the batch's capacity.
 struct bpc_batch {
   int device = 0;
   int max_images = 0;
   long long max_cbs = 0;
-  bpc::BatchImage* d_images = nullptr;
-  bpc::BatchImage* h_images = nullptr;  // pinned staging for the descriptor upload
-  long long* d_offsets = nullptr;       // cb_offset[n+1], sorted_base[n]
-  long long* h_offsets = nullptr;
+  size_t stage_bytes = 0;          // run words + offsets (2*max_images+1) + descriptors (max_images)
+  char* d_stage = nullptr;         // device copy of the staging area
+  char* h_stage = nullptr;         // pinned staging area
   uint32_t* d_keys = nullptr;
   uint32_t* d_sorted = nullptr;
   uint32_t* d_pair_keys = nullptr;
   uint32_t* d_pair_queue = nullptr;
+  uint32_t* d_wide = nullptr;      // pairs deferred to the 28-plane pass
   uint32_t* d_ctl = nullptr;
-  hipEvent_t staged = nullptr;  // the last descriptor upload has been consumed
+  hipEvent_t staged = nullptr;  // the last staging upload has been consumed
   bool staged_pending = false;
   hipEvent_t done = nullptr;  // the last run has finished
   bool done_pending = false;
@@ -813,21 +486,41 @@ struct bpc_batch {
   int waves_per_simd = 0;  // 0: as many as fit
   int vgpr_cap_waves = 0;  // 0: compiler's register budget; 5..8: capped for that many waves per SIMD
   int cus = 0;
-  std::map<std::string, hipGraphExec_t> graphs;
+  // captured launch sequences, least recently used first; at most kMaxGraphs
+  static constexpr size_t kMaxGraphs = 32;
+  std::list<std::pair<std::string, hipGraphExec_t>> graphs;
   std::mutex mu;
+
+  bpc::BatchRun* h_run() { return reinterpret_cast<bpc::BatchRun*>(h_stage); }
+  long long* h_offsets() { return reinterpret_cast<long long*>(h_stage + sizeof(bpc::BatchRun)); }
+  bpc::BatchImage* h_images() {
+    return reinterpret_cast<bpc::BatchImage*>(h_stage + images_offset());
+  }
+  const bpc::BatchRun* d_run() const { return reinterpret_cast<const bpc::BatchRun*>(d_stage); }
+  const long long* d_offsets() const { return reinterpret_cast<const long long*>(d_stage + sizeof(bpc::BatchRun)); }
+  const bpc::BatchImage* d_images() const {
+    return reinterpret_cast<const bpc::BatchImage*>(d_stage + images_offset());
+  }
+  size_t images_offset() const {
+    const size_t o = sizeof(bpc::BatchRun) + sizeof(long long) * (2 * (size_t)max_images + 1);
+    return (o + 63) & ~(size_t)63;
+  }
 };
 
 namespace {
 
-void batch_release(bpc_batch* b) {
+void drop_graphs(bpc_batch* b) {
   for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
   b->graphs.clear();
-  void* dev[] = {b->d_images, b->d_offsets, b->d_keys,  b->d_sorted,
-                 b->d_pair_keys, b->d_pair_queue, b->d_ctl, b->d_scratch};
+}
+
+void batch_release(bpc_batch* b) {
+  drop_graphs(b);
+  void* dev[] = {b->d_stage, b->d_keys, b->d_sorted, b->d_pair_keys, b->d_pair_queue, b->d_wide, b->d_ctl,
+                 b->d_scratch};
   for (void* p : dev)
     if (p) (void)hipFree(p);
-  if (b->h_images) (void)hipHostFree(b->h_images);
-  if (b->h_offsets) (void)hipHostFree(b->h_offsets);
+  if (b->h_stage) (void)hipHostFree(b->h_stage);
   if (b->staged) (void)hipEventDestroy(b->staged);
   if (b->done) (void)hipEventDestroy(b->done);
 }
@@ -836,30 +529,27 @@ struct BatchPlan {
   long long total_cbs = 0, total_pairs = 0;
   int maxp = 16;
   int rows = 0;
-  std::string signature;
+  std::string signature;  // every value the captured launch sequence depends on
 };
 
 int batch_stage(bpc_batch* b, int direction, const bpc_image* images, int n, BatchPlan& plan) {
   if (n <= 0 || n > b->max_images) return fail(BPC_ERR_ARG, "batch of %d images (capacity %d)", n, b->max_images);
-  if (n > 4096) return fail(BPC_ERR_ARG, "at most 4096 images per batch");
   plan.rows = images[0].params.cblock_length;
   int max_nbp = 1;
-  char sig[128];
-  plan.signature = std::to_string(direction) + ":";
+  char sig[160];
+  plan.signature = "d" + std::to_string(direction) + ";n" + std::to_string(n) + ";";
   long long pos = 0;
+  bpc::BatchImage* hi = b->h_images();
+  long long* ho = b->h_offsets();
   for (int k = 0; k < n; ++k) {
     const bpc_image& im = images[k];
-    int st = check_params(&im.params);
-    if (st) return st;
+    const long long ncb = bpc::batch_image_codeblocks(im.size_x, im.size_y, &im.params);
+    if (ncb < 0) return (int)ncb;
     if (im.params.cblock_length != plan.rows) return fail(BPC_ERR_ARG, "all images of a batch share cblock_length");
-    st = check_image(im.size_x, im.size_y, &im.params);
-    if (st) return st;
     if (!im.src || !im.dst || !im.lut) return fail(BPC_ERR_ARG, "image %d: null buffer", k);
     if (((uintptr_t)im.src & 7) || ((uintptr_t)im.dst & 7)) return fail(BPC_ERR_ARG, "image %d: buffers must be 8-byte aligned", k);
     if (im.lut_len < bpc_lut_len(&im.params)) return fail(BPC_ERR_ARG, "image %d: lut_len too small", k);
-    const long long ncb = (long long)(im.size_x / 64) * (im.size_y / plan.rows);
-    if (ncb > (1 << 20)) return fail(BPC_ERR_ARG, "image %d: more than 2^20 codeblocks", k);
-    bpc::BatchImage& d = b->h_images[k];
+    bpc::BatchImage& d = hi[k];
     d.src = im.src;
     d.dst = im.dst;
     d.lut = im.lut;
@@ -876,48 +566,66 @@ int batch_stage(bpc_batch* b, int direction, const bpc_image* images, int n, Bat
     d.fill = im.fill_unused;
     d.cpass3 = im.params.cpass3;
     d.pad = 0;
-    b->h_offsets[k] = plan.total_cbs;
-    b->h_offsets[n + 1 + k] = pos;
+    ho[k] = plan.total_cbs;
+    ho[n + 1 + k] = pos;
     plan.total_cbs += ncb;
     pos += (ncb + 1) & ~1ll;
     max_nbp = std::max(max_nbp, im.params.lut_n_bitplanes);
-    snprintf(sig, sizeof sig, "%dx%d/%d%d%d%d/%d;", im.size_x, im.size_y, im.params.cpass3, im.params.dwt_levels,
-             im.params.lut_n_bitplanes, im.params.layout, d.stride);
+    // geometry, pass mode and bitplanes of every image: the queue build and the
+    // grid depend on them (pointers and LUTs are read from the uploaded descriptors)
+    snprintf(sig, sizeof sig, "%dx%d,c%d,l%d,b%d,o%d,s%d,f%d;", im.size_x, im.size_y, im.params.cpass3,
+             im.params.dwt_levels, im.params.lut_n_bitplanes, im.params.layout, d.stride, im.fill_unused);
     plan.signature += sig;
   }
-  b->h_offsets[n] = plan.total_cbs;
+  ho[n] = plan.total_cbs;
   plan.total_pairs = pos / 2;
   if (plan.total_cbs > b->max_cbs) return fail(BPC_ERR_ARG, "batch of %lld codeblocks (capacity %lld)", plan.total_cbs, b->max_cbs);
   plan.maxp = bpc::planes_for(max_nbp);
+  snprintf(sig, sizeof sig, "r%d,cbs%lld,pairs%lld,p%d", plan.rows, plan.total_cbs, plan.total_pairs, plan.maxp);
+  plan.signature += sig;
   return BPC_OK;
 }
 
-int batch_grid_blocks(bpc_batch* b, int direction, int maxp, const BatchPlan& plan) {
-  if (plan.total_pairs <= 0) return 0;
+// Workgroups of the persistent kernel that fit on the device at once.
+int batch_resident_blocks(bpc_batch* b, int direction, int maxp) {
   const int wpb = b->waves_per_block;
   int per_cu = bpc::batch_resident_blocks_per_cu(direction == BPC_DECODE, maxp, wpb, b->vgpr_cap_waves);
   if (per_cu <= 0) per_cu = 1;
   if (b->waves_per_simd > 0) per_cu = std::max(1, std::min(per_cu, (4 * b->waves_per_simd) / wpb));
-  long long blocks = (long long)per_cu * b->cus;
-  blocks = std::min(blocks, (plan.total_pairs + wpb - 1) / wpb);
-  return (int)blocks;
+  return per_cu * b->cus;
 }
 
-int batch_upload(bpc_batch* b, int n, hipStream_t stream) {
-  HIP_TRY(hipMemcpyAsync(b->d_images, b->h_images, sizeof(bpc::BatchImage) * n, hipMemcpyHostToDevice, stream));
-  HIP_TRY(hipMemcpyAsync(b->d_offsets, b->h_offsets, sizeof(long long) * (2 * n + 1), hipMemcpyHostToDevice, stream));
+// Workgroups of the persistent kernel: resident capacity, at most one wave per pair.
+int batch_grid_blocks(bpc_batch* b, int direction, int maxp, const BatchPlan& plan) {
+  if (plan.total_pairs <= 0) return 0;
+  const int wpb = b->waves_per_block;
+  return (int)std::min<long long>(batch_resident_blocks(b, direction, maxp), (plan.total_pairs + wpb - 1) / wpb);
+}
+
+// Workgroups of the 28-plane pass over the deferred pairs (rare: a codeblock whose
+// MSB is at least the main pass's planes): one per CU.
+int batch_wide_blocks(bpc_batch* b, int maxp, const BatchPlan& plan) {
+  if (maxp >= 28 || plan.total_pairs <= 0) return 0;
+  return (int)std::max<long long>(1, std::min<long long>(b->cus, (plan.total_pairs + b->waves_per_block - 1) /
+                                                                     b->waves_per_block));
+}
+
+int batch_upload(bpc_batch* b, int n, uint32_t* d_status, hipStream_t stream) {
+  b->h_run()->status = d_status;
+  const size_t bytes = b->images_offset() + sizeof(bpc::BatchImage) * (size_t)n;
+  HIP_TRY(hipMemcpyAsync(b->d_stage, b->h_stage, bytes, hipMemcpyHostToDevice, stream));
   HIP_TRY(hipEventRecord(b->staged, stream));
   b->staged_pending = true;
   return BPC_OK;
 }
 
 // Kernel sequence of one batch run (captured into a graph when asked).
-int batch_kernels(bpc_batch* b, int direction, int n, const BatchPlan& plan, int maxp, int blocks,
-                  uint32_t* d_status, hipStream_t stream) {
+int batch_kernels(bpc_batch* b, int direction, int n, const BatchPlan& plan, int maxp, int blocks, int wide_blocks,
+                  hipStream_t stream) {
   bpc::QueueArgs q;
-  q.images = b->d_images;
-  q.cb_offset = b->d_offsets;
-  q.sorted_base = b->d_offsets + n + 1;
+  q.images = b->d_images();
+  q.cb_offset = b->d_offsets();
+  q.sorted_base = b->d_offsets() + n + 1;
   q.n_images = n;
   q.total_cbs = plan.total_cbs;
   q.total_pairs = plan.total_pairs;
@@ -932,17 +640,25 @@ int batch_kernels(bpc_batch* b, int direction, int n, const BatchPlan& plan, int
   if (e != hipSuccess) return hip_fail(e, "queue build");
   bpc::BatchArgs a;
   a.scratch = b->d_scratch;
-  a.images = b->d_images;
+  a.images = b->d_images();
   a.sorted = b->d_sorted;
   a.pair_queue = b->d_pair_queue;
+  a.pair_keys = b->d_pair_keys;
+  a.wide = b->d_wide;
   a.ctl = b->d_ctl;
   a.n_pairs = plan.total_pairs;
   a.rows = plan.rows;
+  a.wide_pass = 0;
   if (blocks > 0) {
     e = bpc::launch_batch(direction == BPC_DECODE, maxp, a, blocks, b->waves_per_block, b->vgpr_cap_waves, stream);
     if (e != hipSuccess) return hip_fail(e, "batched kernel launch");
   }
-  e = bpc::launch_status_fold(b->d_ctl, d_status, stream);
+  if (wide_blocks > 0) {
+    a.wide_pass = 1;
+    e = bpc::launch_batch(direction == BPC_DECODE, 28, a, wide_blocks, b->waves_per_block, 0, stream);
+    if (e != hipSuccess) return hip_fail(e, "batched kernel launch (28 planes)");
+  }
+  e = bpc::launch_status_fold(b->d_ctl, b->d_run(), stream);
   if (e != hipSuccess) return hip_fail(e, "status fold");
   return BPC_OK;
 }
@@ -952,8 +668,9 @@ int batch_kernels(bpc_batch* b, int direction, int n, const BatchPlan& plan, int
 extern "C" {
 
 int bpc_batch_create(bpc_batch** out, int max_images, long long max_codeblocks) {
-  if (!out || max_images <= 0 || max_images > 4096 || max_codeblocks <= 0)
-    return fail(BPC_ERR_ARG, "bad batch capacity");
+  if (!out || max_images <= 0 || max_images > bpc::MAX_BATCH_IMAGES || max_codeblocks <= 0 ||
+      max_codeblocks > ((long long)1 << 32))
+    return fail(BPC_ERR_ARG, "bad batch capacity (1..%d images, 1..2^32 codeblocks)", bpc::MAX_BATCH_IMAGES);
   bpc_batch* b = new bpc_batch();
   b->max_images = max_images;
   b->max_cbs = max_codeblocks;
@@ -962,14 +679,14 @@ int bpc_batch_create(bpc_batch** out, int max_images, long long max_codeblocks) 
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
   b->cus = cus > 0 ? cus : 1;
   const size_t entries = (size_t)max_codeblocks + 2 * (size_t)max_images + 2;
-  if (e == hipSuccess) e = hipMalloc(&b->d_images, sizeof(bpc::BatchImage) * max_images);
-  if (e == hipSuccess) e = hipHostMalloc(&b->h_images, sizeof(bpc::BatchImage) * max_images, hipHostMallocDefault);
-  if (e == hipSuccess) e = hipMalloc(&b->d_offsets, sizeof(long long) * (2 * max_images + 1));
-  if (e == hipSuccess) e = hipHostMalloc(&b->h_offsets, sizeof(long long) * (2 * max_images + 1), hipHostMallocDefault);
+  b->stage_bytes = b->images_offset() + sizeof(bpc::BatchImage) * (size_t)max_images;
+  if (e == hipSuccess) e = hipMalloc(&b->d_stage, b->stage_bytes);
+  if (e == hipSuccess) e = hipHostMalloc(&b->h_stage, b->stage_bytes, hipHostMallocDefault);
   if (e == hipSuccess) e = hipMalloc(&b->d_keys, sizeof(uint32_t) * max_codeblocks);
   if (e == hipSuccess) e = hipMalloc(&b->d_sorted, sizeof(uint32_t) * entries);
   if (e == hipSuccess) e = hipMalloc(&b->d_pair_keys, sizeof(uint32_t) * (entries / 2 + 1));
   if (e == hipSuccess) e = hipMalloc(&b->d_pair_queue, sizeof(uint32_t) * (entries / 2 + 1));
+  if (e == hipSuccess) e = hipMalloc(&b->d_wide, sizeof(uint32_t) * (entries / 2 + 1));
   if (e == hipSuccess) e = hipMalloc(&b->d_ctl, sizeof(uint32_t) * (bpc::CTL_IMG + 32 * (size_t)max_images));
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->staged, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
@@ -978,6 +695,7 @@ int bpc_batch_create(bpc_batch** out, int max_images, long long max_codeblocks) 
     delete b;
     return hip_fail(e, "batch buffers");
   }
+  memset(b->h_stage, 0, b->stage_bytes);
   *out = b;
   return BPC_OK;
 }
@@ -985,6 +703,7 @@ int bpc_batch_create(bpc_batch** out, int max_images, long long max_codeblocks) 
 int bpc_batch_destroy(bpc_batch* b) {
   if (!b) return BPC_OK;
   if (b->done_pending) (void)hipEventSynchronize(b->done);
+  if (b->staged_pending) (void)hipEventSynchronize(b->staged);
   batch_release(b);
   delete b;
   return BPC_OK;
@@ -1007,6 +726,12 @@ int bpc_batch_set_vgpr_cap(bpc_batch* b, int waves_per_simd) {
   return BPC_OK;
 }
 
+int bpc_batch_graph_count(bpc_batch* b) {
+  if (!b) return fail(BPC_ERR_ARG, "null batch");
+  std::lock_guard<std::mutex> lk(b->mu);
+  return (int)b->graphs.size();
+}
+
 int bpc_batch_run(bpc_batch* b, int direction, const bpc_image* images, int n_images, int use_graph,
                   uint32_t* d_status, void* stream) {
   if (!b || !images) return fail(BPC_ERR_ARG, "null batch or images");
@@ -1024,28 +749,37 @@ int bpc_batch_run(bpc_batch* b, int direction, const bpc_image* images, int n_im
   hipStream_t s = (hipStream_t)stream;
   const int maxp = plan.maxp;
   const int blocks = batch_grid_blocks(b, direction, maxp, plan);
-  const size_t need = (size_t)blocks * b->waves_per_block * bpc::scratch_bytes_per_wave(maxp);
-  if (need > b->scratch_bytes) {  // grow the scratch once the previous run is done
+  const int wide_blocks = batch_wide_blocks(b, maxp, plan);
+  const size_t need = std::max((size_t)blocks * b->waves_per_block * bpc::scratch_bytes_per_wave(maxp),
+                               (size_t)wide_blocks * b->waves_per_block * bpc::scratch_bytes_per_wave(28));
+  if (need > b->scratch_bytes) {
+    // Grow once the previous run is done, to what the full resident grid of this
+    // tuning needs, so that larger batches later do not grow it again (a growth
+    // drops the captured graphs, which hold the old address).
+    const size_t full = std::max((size_t)batch_resident_blocks(b, direction, maxp) * b->waves_per_block *
+                                     bpc::scratch_bytes_per_wave(maxp),
+                                 (size_t)b->cus * b->waves_per_block * bpc::scratch_bytes_per_wave(28));
     if (b->done_pending) HIP_TRY(hipEventSynchronize(b->done));
-    for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
-    b->graphs.clear();
+    drop_graphs(b);
     if (b->d_scratch) HIP_TRY(hipFree(b->d_scratch));
     b->d_scratch = nullptr;
     b->scratch_bytes = 0;
-    HIP_TRY(hipMalloc(&b->d_scratch, need));
-    b->scratch_bytes = need;
+    HIP_TRY(hipMalloc(&b->d_scratch, std::max(need, full)));
+    b->scratch_bytes = std::max(need, full);
   }
-  st = batch_upload(b, n_images, s);
+  st = batch_upload(b, n_images, d_status, s);
   if (st) return st;
   if (!use_graph) {
-    st = batch_kernels(b, direction, n_images, plan, maxp, blocks, d_status, s);
+    st = batch_kernels(b, direction, n_images, plan, maxp, blocks, wide_blocks, s);
   } else {
     char tun[96];
-    snprintf(tun, sizeof tun, "|%d,%d,%d,%d,%d,%p", b->waves_per_block, b->waves_per_simd, b->vgpr_cap_waves, maxp, blocks,
-             (void*)d_status);
+    snprintf(tun, sizeof tun, "|wpb%d,wps%d,cap%d,g%d,w%d", b->waves_per_block, b->waves_per_simd, b->vgpr_cap_waves,
+             blocks, wide_blocks);
     const std::string key = plan.signature + tun;
-    auto it = b->graphs.find(key);
-    if (it == b->graphs.end()) {
+    auto it = std::find_if(b->graphs.begin(), b->graphs.end(), [&](const auto& kv) { return kv.first == key; });
+    if (it != b->graphs.end()) {
+      b->graphs.splice(b->graphs.end(), b->graphs, it);  // most recently used last
+    } else {
       hipStream_t cap = nullptr;
       HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
       hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
@@ -1053,7 +787,7 @@ int bpc_batch_run(bpc_batch* b, int direction, const bpc_image* images, int n_im
         (void)hipStreamDestroy(cap);
         return hip_fail(e, "graph capture");
       }
-      st = batch_kernels(b, direction, n_images, plan, maxp, blocks, d_status, cap);
+      st = batch_kernels(b, direction, n_images, plan, maxp, blocks, wide_blocks, cap);
       hipGraph_t graph = nullptr;
       e = hipStreamEndCapture(cap, &graph);
       (void)hipStreamDestroy(cap);
@@ -1066,9 +800,14 @@ int bpc_batch_run(bpc_batch* b, int direction, const bpc_image* images, int n_im
       e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
       if (e != hipSuccess) return hip_fail(e, "graph instantiate");
-      it = b->graphs.emplace(key, exec).first;
+      if (b->graphs.size() >= bpc_batch::kMaxGraphs) {  // evict the least recently used, once idle
+        if (b->done_pending) HIP_TRY(hipEventSynchronize(b->done));
+        (void)hipGraphExecDestroy(b->graphs.front().second);
+        b->graphs.pop_front();
+      }
+      b->graphs.emplace_back(key, exec);
     }
-    HIP_TRY(hipGraphLaunch(it->second, s));
+    HIP_TRY(hipGraphLaunch(b->graphs.back().second, s));
   }
   if (st) return st;
   HIP_TRY(hipEventRecord(b->done, s));
@@ -1128,106 +867,6 @@ int bpc_compact_unpack_device(const uint8_t* d_msb, const uint64_t* d_offsets, c
   hipError_t e = bpc::launch_compact_unpack(d_msb, reinterpret_cast<const unsigned long long*>(d_offsets), d_payload,
                                             payload_len, ncb, stride_words, d_stream, d_status, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e, "compact unpack launch");
-  return BPC_OK;
-}
-
-// load_Data's codeblock file order (main_example.cu:71-179): the LL band, then per
-// level from the coarsest the HL, LH and HH bands, each in row-major codeblock steps.
-int bpc_cb_file_order(int size_x, int size_y, int dwt_levels, int cblock_length, int32_t* tile_of_file, int cap) {
-  if (size_x <= 0 || size_y <= 0 || size_x % 64 || cblock_length < 2 || cblock_length > 64 ||
-      size_y % cblock_length || dwt_levels < 0 || dwt_levels > 16 || !tile_of_file)
-    return fail(BPC_ERR_ARG, "bad image geometry");
-  const int nx = size_x / 64;
-  const int ncb = nx * (size_y / cblock_length);
-  if (cap < ncb) return fail(BPC_ERR_ARG, "capacity %d < %d codeblocks", cap, ncb);
-  std::vector<char> seen(ncb, 0);
-  int k = 0;
-  auto band = [&](int x0, int y0, int sx, int sy) -> bool {
-    if (sx % 64 || sy % cblock_length) return false;
-    for (int j = 0; j < sy; j += cblock_length)
-      for (int i = 0; i < sx; i += 64) {
-        const int t = ((y0 + j) / cblock_length) * nx + (x0 + i) / 64;
-        if (k >= ncb || seen[t]) return false;
-        seen[t] = 1;
-        tile_of_file[k++] = t;
-      }
-    return true;
-  };
-  bool ok = band(0, 0, size_x >> dwt_levels, size_y >> dwt_levels);
-  for (int d = dwt_levels; ok && d > 0; --d) {
-    const int sx = size_x >> d, sy = size_y >> d;
-    ok = band(sx, 0, sx, sy) && band(0, sy, sx, sy) && band(sx, sy, sx, sy);
-  }
-  if (!ok || k != ncb)
-    return fail(BPC_ERR_ARG, "%dx%d with %d levels: subbands are not whole 64x%d codeblocks (no file order)", size_x,
-                size_y, dwt_levels, cblock_length);
-  return k;
-}
-
-// One file per codeblock, named by its index in the file order: the MSB and the
-// codewords as big-endian 16-bit values (what load_CStream, main_example.cu:51-68,
-// reads back into the reference layout, every word past the end reading as -1).
-int bpc_bin_export(const char* folder, const uint8_t* msb, const uint64_t* offsets, const uint16_t* payload,
-                   int size_x, int size_y, const bpc_params* p) {
-  int st = check_params(p);
-  if (st) return st;
-  if (!folder || !msb || !offsets) return fail(BPC_ERR_ARG, "null argument");
-  const int ncb = (size_x / 64) * (size_y / p->cblock_length);
-  std::vector<int32_t> order(ncb > 0 ? ncb : 1);
-  int n = bpc_cb_file_order(size_x, size_y, p->dwt_levels, p->cblock_length, order.data(), ncb);
-  if (n < 0) return n;
-  std::string dir(folder);
-  if (!dir.empty() && dir.back() != '/') dir += '/';
-  std::vector<uint8_t> buf;
-  for (int k = 0; k < n; ++k) {
-    const int t = order[k];
-    const uint64_t o = offsets[t], e = offsets[t + 1];
-    if (e < o || e - o > (uint64_t)stride_of(p) - 1) return fail(BPC_ERR_ARG, "codeblock %d: bad offsets", t);
-    buf.resize(2 * (1 + (e - o)));
-    buf[0] = 0;
-    buf[1] = msb[t];
-    for (uint64_t j = o; j < e; ++j) {
-      buf[2 * (1 + j - o)] = (uint8_t)(payload[j] >> 8);
-      buf[2 * (1 + j - o) + 1] = (uint8_t)(payload[j] & 0xFF);
-    }
-    const std::string path = dir + std::to_string(k) + ".bin";
-    FILE* f = fopen(path.c_str(), "wb");
-    if (!f) return fail(BPC_ERR_ARG, "cannot write %s", path.c_str());
-    const size_t w = fwrite(buf.data(), 1, buf.size(), f);
-    const int c = fclose(f);
-    if (w != buf.size() || c != 0) return fail(BPC_ERR_ARG, "short write to %s", path.c_str());
-  }
-  return BPC_OK;
-}
-
-// load_Data with CStreamDataLoad (main_example.cu:71-179, 51-68): file k fills the
-// region of tile order[k] with stride big-endian 16-bit words, -1 past the end.
-int bpc_bin_import(const char* folder, int size_x, int size_y, const bpc_params* p, int32_t* host_stream) {
-  int st = check_params(p);
-  if (st) return st;
-  if (!folder || !host_stream) return fail(BPC_ERR_ARG, "null argument");
-  const int ncb = (size_x / 64) * (size_y / p->cblock_length);
-  std::vector<int32_t> order(ncb > 0 ? ncb : 1);
-  int n = bpc_cb_file_order(size_x, size_y, p->dwt_levels, p->cblock_length, order.data(), ncb);
-  if (n < 0) return n;
-  std::string dir(folder);
-  if (!dir.empty() && dir.back() != '/') dir += '/';
-  const int stride = stride_of(p);
-  std::vector<uint8_t> buf((size_t)2 * stride);
-  for (int k = 0; k < n; ++k) {
-    const std::string path = dir + std::to_string(k) + ".bin";
-    FILE* f = fopen(path.c_str(), "rb");
-    if (!f) return fail(BPC_ERR_ARG, "cannot read %s", path.c_str());
-    const size_t got = fread(buf.data(), 1, buf.size(), f);
-    fclose(f);
-    int32_t* region = host_stream + (int64_t)order[k] * stride;
-    for (int i = 0; i < stride; ++i) {
-      const size_t b = 2 * (size_t)i;
-      const uint32_t hi = b < got ? buf[b] : 0xFFFFFFFFu;
-      const uint32_t lo = b + 1 < got ? buf[b + 1] : 0xFFFFFFFFu;
-      region[i] = (int32_t)((hi << 8) | lo);
-    }
-  }
   return BPC_OK;
 }
 

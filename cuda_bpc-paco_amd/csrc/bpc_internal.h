@@ -3,16 +3,9 @@
 #include <stdint.h>
 #include <hip/hip_runtime_api.h>
 
-namespace bpc {
+#include "bpc_host.h"
 
-// Device status bits (OR-ed by the kernels into a status word).
-enum : uint32_t {
-  ST_RANGE = 1u,      // |coefficient| >= 2^28, LUT entry outside [0,127] or LUT row outside the array
-  ST_OVERFLOW = 2u,   // codeword slots past the codeblock region
-  ST_MALFORMED = 4u,  // decoder: MSB word not in [0,27] and not 32
-  ST_WIDE = 8u,       // a codeblock needs more bitplanes than the instantiated kernel holds
-  ST_NARROW = 16u,    // decoder: a coefficient outside the int16 output range (saturated)
-};
+namespace bpc {
 
 // One launch over the codeblocks of one image.
 struct KArgs {
@@ -47,14 +40,22 @@ struct BatchImage {
 
 // Control words of a batch queue (device memory): CTL_IMG + 32*n_images words.
 enum : int {
-  CTL_PAIRHIST = 0,  // 58 pair-bucket counts, then cursors
-  CTL_HEAD = 64,     // next queue slot to hand out
-  CTL_STATUS = 65,   // status bits of the batch
-  CTL_IMG = 128,     // per image: 29 MSB-bucket counts, then cursors (32 words each)
+  CTL_PAIRHIST = 0,     // 58 pair-bucket counts, then cursors
+  CTL_HEAD = 64,        // next queue slot to hand out
+  CTL_STATUS = 65,      // status bits of the batch
+  CTL_WIDE_COUNT = 66,  // pairs deferred to the 28-plane pass (MSB beyond the main pass's planes)
+  CTL_WIDE_HEAD = 67,   // next deferred pair to hand out
+  CTL_IMG = 128,        // per image: 29 MSB-bucket counts, then cursors (32 words each)
 };
 
-constexpr uint32_t QUEUE_EMPTY = 0xFFFFFFFFu;  // the missing partner of an odd codeblock count
-constexpr int NBUCKET = 29;                      // MSB buckets: 27..0, then all-zero / rejected
+constexpr int NBUCKET = 29;  // MSB buckets: 27..0, then all-zero / rejected
+
+// Per-run words of a batch, uploaded with the image descriptors (so that a
+// captured launch sequence reads them at replay time).
+struct BatchRun {
+  uint32_t* status;  // the caller's status word
+  uint64_t pad[7];
+};
 
 struct QueueArgs {
   const BatchImage* images;
@@ -77,9 +78,12 @@ struct BatchArgs {
   const BatchImage* images;
   const uint32_t* sorted;
   const uint32_t* pair_queue;
+  const uint32_t* pair_keys;  // per pair: pass mode * 29 + MSB bucket of its costlier codeblock
+  uint32_t* wide;             // pairs deferred to the 28-plane pass
   uint32_t* ctl;
   long long n_pairs;
   int rows;
+  int wide_pass;  // 0: drain the pair queue (deferring pairs beyond the kernel's planes); 1: drain `wide`
 };
 
 // Symbol statistics of a training image (LUT trainer).
@@ -103,7 +107,7 @@ hipError_t launch_queue_build(const QueueArgs& a, hipStream_t stream);
 // vgpr_cap_waves: 0 (compiler's register budget) or 5..8 (registers capped for that many waves per SIMD)
 hipError_t launch_batch(int decode, int maxp, const BatchArgs& a, int grid_blocks, int waves_per_block,
                         int vgpr_cap_waves, hipStream_t stream);
-hipError_t launch_status_fold(uint32_t* ctl, uint32_t* status, hipStream_t stream);
+hipError_t launch_status_fold(uint32_t* ctl, const BatchRun* run, hipStream_t stream);
 // Workgroups of the batched kernel that fit on one CU (occupancy query).
 int batch_resident_blocks_per_cu(int decode, int maxp, int waves_per_block, int vgpr_cap_waves);
 // Trainer workgroups that fit on one CU with lds_bytes of LDS each.

@@ -990,15 +990,29 @@ void bpc_batch_kernel(const BatchArgs a) {
   const int half = (threadIdx.x & 63) >> 5;
   const size_t wave = (size_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   void* scr = reinterpret_cast<char*>(a.scratch) + wave * (2 * MAXP * 64 * 8);
+  // The main pass drains the pair queue and defers the pairs whose MSB is beyond
+  // its planes; a 28-plane pass then drains the deferred pairs.
+  const bool wide_pass = a.wide_pass != 0;
+  const u32* queue = wide_pass ? a.wide : a.pair_queue;
+  u32* head = &a.ctl[wide_pass ? CTL_WIDE_HEAD : CTL_HEAD];
+  const u32 n_items = wide_pass ? (u32)__builtin_amdgcn_readfirstlane((int)a.ctl[CTL_WIDE_COUNT]) : (u32)a.n_pairs;
   for (;;) {
     u32 slot = 0;
-    if ((threadIdx.x & 63) == 0) slot = atomicAdd(&a.ctl[CTL_HEAD], 1u);
+    if ((threadIdx.x & 63) == 0) slot = atomicAdd(head, 1u);
     slot = __builtin_amdgcn_readfirstlane(slot);
-    if (slot >= (u32)a.n_pairs) break;
-    const u32 pair = __builtin_amdgcn_readfirstlane(a.pair_queue[slot]);
+    if (slot >= n_items) break;
+    const u32 pair = __builtin_amdgcn_readfirstlane(queue[slot]);
+    if (MAXP < 28 && !wide_pass) {
+      // the pair's costlier codeblock comes first: its MSB bucket is the pair's key
+      const u32 b = (u32)__builtin_amdgcn_readfirstlane((int)a.pair_keys[pair]) % (u32)NBUCKET;
+      if (b < (u32)NBUCKET - 1u && 27 - (int)b >= MAXP) {
+        if ((threadIdx.x & 63) == 0) a.wide[atomicAdd(&a.ctl[CTL_WIDE_COUNT], 1u)] = pair;
+        continue;
+      }
+    }
     const u32 first = __builtin_amdgcn_readfirstlane(a.sorted[2 * (size_t)pair]);
     const u32 e = a.sorted[2 * (size_t)pair + half];
-    const BatchImage& im = a.images[first >> 20];
+    const BatchImage& im = a.images[queue_entry_image(first)];
     HalfDesc d;
     d.src = (cgi32)im.src;
     d.dst = (gi32)im.dst;
@@ -1016,7 +1030,7 @@ void bpc_batch_kernel(const BatchArgs a) {
     d.in16 = 0;
     d.slut = sl;
     d.prows = pr;
-    d.cb = e == QUEUE_EMPTY ? -1 : (int)(e & 0xFFFFFu);
+    d.cb = e == QUEUE_EMPTY ? -1 : (int)queue_entry_cb(e);
     if (im.cpass3)
       code_pair<DEC, 1, MAXP>(d, a.rows, &a.ctl[CTL_STATUS], scr);
     else

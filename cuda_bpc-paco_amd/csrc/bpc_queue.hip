@@ -146,7 +146,7 @@ __global__ __launch_bounds__(256) void queue_scatter_kernel(const QueueArgs a) {
     if (a.cb_offset[mid] <= gcb) lo = mid; else hi = mid - 1;
   }
   const u32 pos = wave_atomic_inc(a.ctl + CTL_IMG, (u32)lo * 32u + a.keys[gcb]);
-  a.sorted[pos] = ((u32)lo << 20) | (u32)(gcb - a.cb_offset[lo]);
+  a.sorted[pos] = queue_entry((u32)lo, (u32)(gcb - a.cb_offset[lo]));
 }
 
 // One thread per pair: key (pass mode, bucket of its costlier codeblock).
@@ -154,8 +154,8 @@ __global__ __launch_bounds__(256) void queue_pair_keys_kernel(const QueueArgs a)
   const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= a.total_pairs) return;
   const u32 e = a.sorted[2 * p];
-  const int img = (int)(e >> 20);
-  const u32 b = a.keys[a.cb_offset[img] + (e & 0xFFFFFu)];
+  const int img = (int)queue_entry_image(e);
+  const u32 b = a.keys[a.cb_offset[img] + queue_entry_cb(e)];
   const u32 k = (a.images[img].cpass3 ? NBUCKET : 0) + b;
   a.pair_keys[p] = k;
   (void)wave_atomic_inc(a.ctl + CTL_PAIRHIST, k);
@@ -178,9 +178,10 @@ __global__ __launch_bounds__(256) void queue_pair_scatter_kernel(const QueueArgs
   a.pair_queue[pos] = (u32)p;
 }
 
-// Folds the batch's status into the caller's status word.
-__global__ void status_fold_kernel(u32* ctl, uint32_t* status) {
-  if (threadIdx.x == 0 && ctl[CTL_STATUS]) atomicOr(status, ctl[CTL_STATUS]);
+// Folds the batch's status into the caller's status word (named by the run's
+// uploaded words, so that a replayed graph folds into the current caller's word).
+__global__ void status_fold_kernel(u32* ctl, const BatchRun* run) {
+  if (threadIdx.x == 0 && ctl[CTL_STATUS]) atomicOr(run->status, ctl[CTL_STATUS]);
 }
 
 }  // namespace
@@ -204,8 +205,8 @@ hipError_t launch_queue_build(const QueueArgs& a, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_status_fold(uint32_t* ctl, uint32_t* status, hipStream_t stream) {
-  hipLaunchKernelGGL(status_fold_kernel, dim3(1), dim3(64), 0, stream, ctl, status);
+hipError_t launch_status_fold(uint32_t* ctl, const BatchRun* run, hipStream_t stream) {
+  hipLaunchKernelGGL(status_fold_kernel, dim3(1), dim3(64), 0, stream, ctl, run);
   return hipGetLastError();
 }
 

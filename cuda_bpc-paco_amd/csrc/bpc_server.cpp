@@ -1,9 +1,10 @@
 // Continuous-batching request server over the batched codec.
 //
 // Requests (encode or decode of one image, device-resident inputs) are queued on
-// arrival.  A scheduler thread admits them in arrival order while the HBM budget
-// allows their output buffers, groups admitted requests of one direction into a
-// batch and launches it on one of several streams.  Up to `max_inflight` batches
+// arrival.  A scheduler thread admits them in arrival order while the device's
+// free memory (less a reserve) and the optional output budget allow their output
+// buffers, groups admitted requests of one direction into a batch and launches it
+// on one of several streams.  Up to `max_inflight` batches
 // run on the device at once, so requests that arrive while others are on the GPU
 // start as soon as a slot and memory free up, and their work overlaps on the
 // device.  Each request's output is the same as coding its image alone.
@@ -21,6 +22,7 @@
 #include <vector>
 
 #include "../../include/bpc_paco.h"
+#include "bpc_host.h"
 
 namespace {
 
@@ -34,7 +36,7 @@ struct Req {
   int32_t* ncw = nullptr;
   Clock::time_point t_submit, t_start, t_done;
   int status = BPC_OK;
-  bool admitted = false;  // its output bytes count against the HBM budget
+  bool admitted = false;  // its output bytes count as committed
   bool done = false;
 };
 
@@ -64,10 +66,13 @@ struct bpc_server {
   std::map<uint64_t, Req*> all;
   uint64_t next_id = 1;
   size_t committed = 0;  // bytes of outputs held by admitted, not yet released requests
+  bpc::AdmissionRule rule;
+  size_t total_mem = 0;  // device memory (mem probe at create)
   // released output buffers kept for reuse, by size (hipFree would synchronize the device)
   std::multimap<size_t, void*> pool;
   size_t pooled = 0;
   std::atomic<bool> stop{false};
+  uint64_t changes = 0;  // submits and releases (what can make a pending request admissible)
   std::thread th;
   // statistics
   uint64_t batches = 0, completed = 0;
@@ -93,23 +98,85 @@ void finish_slot(bpc_server* s, Slot& sl) {
   s->cv_done.notify_all();
 }
 
+size_t ncw_bytes(const bpc_request& r) { return sizeof(int32_t) * (size_t)std::max<int64_t>(1, ncb_of(r)); }
+
+// Device free memory as the admission rule sees it (the injected probe, or HIP).
+size_t free_memory(bpc_server* s) {
+  size_t fr = 0, tot = 0;
+  const int rc = s->cfg.mem_info ? s->cfg.mem_info(s->cfg.mem_info_ctx, &fr, &tot) : (int)hipMemGetInfo(&fr, &tot);
+  return rc == 0 ? fr : 0;
+}
+
+// Takes a pooled buffer of exactly `bytes`, or null (caller holds s->mu).
+void* pool_take(bpc_server* s, size_t bytes) {
+  auto it = s->pool.find(bytes);
+  if (it == s->pool.end()) return nullptr;
+  void* p = it->second;
+  s->pool.erase(it);
+  s->pooled -= bytes;
+  return p;
+}
+
+// Bytes of a request's buffers that the pool cannot serve (caller holds s->mu).
+size_t new_bytes_of(bpc_server* s, const Req* q) {
+  const size_t ob = q->out_bytes, cb = q->r.direction == BPC_CODE ? ncw_bytes(q->r) : 0;
+  const size_t n_out = s->pool.count(ob);
+  size_t need = n_out ? 0 : ob;
+  if (cb && s->pool.count(cb) < 1 + (cb == ob && n_out ? 1u : 0u)) need += cb;
+  return need;
+}
+
+// Frees pooled buffers, largest first, until `bytes` are released (caller holds s->mu;
+// hipFree waits for the device, which only happens under memory pressure).
+void trim_pool(bpc_server* s, size_t bytes) {
+  size_t freed = 0;
+  while (freed < bytes && !s->pool.empty()) {
+    auto it = std::prev(s->pool.end());
+    freed += it->first;
+    s->pooled -= it->first;
+    (void)hipFree(it->second);
+    s->pool.erase(it);
+  }
+}
+
 // Takes admissible requests of one direction off the queue (arrival order); the
-// first pending request fixes the direction and the codeblock length.
+// first pending request fixes the direction and the codeblock length.  Admission
+// stops at the first request of that kind that does not fit (device free memory
+// less the reserve, the output budget, the batch's codeblocks), so that requests
+// are admitted in the order they arrived.  An admitted request claims pooled
+// buffers of its sizes at once (caller holds s->mu).
 std::vector<Req*> take_batch(bpc_server* s) {
   std::vector<Req*> out;
   if (s->pending.empty()) return out;
   const int dir = s->pending.front()->r.direction;
   const int rows = s->pending.front()->r.params.cblock_length;
   int64_t cbs = 0;
+  size_t free_now = 0;
+  bool probed = false;
   for (auto it = s->pending.begin(); it != s->pending.end() && (int)out.size() < s->cfg.max_batch_images;) {
     Req* q = *it;
-    const int64_t n = ncb_of(q->r);
-    const bool fits_mem = s->committed + q->out_bytes <= s->cfg.hbm_budget_bytes;
     if (q->r.direction != dir || q->r.params.cblock_length != rows) {
       ++it;
       continue;
     }
-    if (!fits_mem || cbs + n > s->cfg.max_batch_codeblocks) break;  // admission in arrival order
+    const int64_t n = ncb_of(q->r);
+    if (cbs + n > s->cfg.max_batch_codeblocks) break;
+    size_t nb = new_bytes_of(s, q);
+    if (nb && !probed) {
+      free_now = free_memory(s);
+      probed = true;
+    }
+    bpc::Admit d = s->rule.decide(s->committed, q->out_bytes, nb, free_now, s->pooled);
+    if (d == bpc::Admit::TRIM_POOL) {
+      trim_pool(s, nb + s->rule.reserve - std::min(free_now, nb + s->rule.reserve));
+      nb = new_bytes_of(s, q);
+      free_now = free_memory(s);
+      d = s->rule.decide(s->committed, q->out_bytes, nb, free_now, 0);
+    }
+    if (d != bpc::Admit::YES) break;  // admission in arrival order
+    free_now -= std::min(free_now, nb);  // what this request will allocate
+    q->out = pool_take(s, q->out_bytes);
+    if (q->r.direction == BPC_CODE) q->ncw = (int32_t*)pool_take(s, ncw_bytes(q->r));
     cbs += n;
     s->committed += q->out_bytes;
     q->admitted = true;
@@ -119,28 +186,10 @@ std::vector<Req*> take_batch(bpc_server* s) {
   return out;
 }
 
-size_t ncw_bytes(const bpc_request& r) { return sizeof(int32_t) * (size_t)std::max<int64_t>(1, ncb_of(r)); }
-
-void* pool_get(bpc_server* s, size_t bytes) {
-  {
-    std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->pool.find(bytes);
-    if (it != s->pool.end()) {
-      void* p = it->second;
-      s->pool.erase(it);
-      s->pooled -= bytes;
-      return p;
-    }
-  }
-  void* p = nullptr;
-  if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
-  return p;
-}
-
 // caller holds s->mu
 void pool_put(bpc_server* s, void* p, size_t bytes) {
   if (!p) return;
-  if (s->pooled + bytes > s->cfg.hbm_budget_bytes) {
+  if (s->pooled + bytes > s->cfg.pool_limit_bytes) {
     (void)hipFree(p);
     return;
   }
@@ -153,10 +202,14 @@ int launch_batch(bpc_server* s, Slot& sl, std::vector<Req*>& reqs) {
   const auto now = Clock::now();
   for (size_t k = 0; k < reqs.size(); ++k) {
     Req* q = reqs[k];
-    if (!q->out) q->out = pool_get(s, q->out_bytes);
-    if (!q->out) return BPC_ERR_HIP;
-    if (q->r.direction == BPC_CODE && !q->ncw) q->ncw = (int32_t*)pool_get(s, ncw_bytes(q->r));
-    if (q->r.direction == BPC_CODE && !q->ncw) return BPC_ERR_HIP;
+    if (!q->out && hipMalloc(&q->out, q->out_bytes) != hipSuccess) {
+      q->out = nullptr;
+      return bpc::fail(BPC_ERR_HIP, "server: output allocation of %zu bytes", q->out_bytes);
+    }
+    if (q->r.direction == BPC_CODE && !q->ncw && hipMalloc(&q->ncw, ncw_bytes(q->r)) != hipSuccess) {
+      q->ncw = nullptr;
+      return bpc::fail(BPC_ERR_HIP, "server: count allocation");
+    }
     q->t_start = now;
     bpc_image& im = imgs[k];
     im.src = q->r.src;
@@ -216,10 +269,14 @@ void scheduler(bpc_server* s) {
       std::unique_lock<std::mutex> lk(s->mu);
       bool any_busy = false;
       for (Slot& sl : s->slots) any_busy |= sl.busy;
-      if (any_busy)
-        s->cv_work.wait_for(lk, std::chrono::microseconds(20));
-      else
-        s->cv_work.wait_for(lk, std::chrono::milliseconds(5), [&] { return s->stop.load() || !s->pending.empty(); });
+      if (any_busy) {
+        s->cv_work.wait_for(lk, std::chrono::microseconds(20));  // completion polling
+      } else {
+        // nothing in flight: wait for a submit or a release; device memory freed by
+        // others is seen at the next probe, at most 1 ms later
+        const uint64_t seen = s->changes;
+        s->cv_work.wait_for(lk, std::chrono::milliseconds(1), [&] { return s->stop.load() || s->changes != seen; });
+      }
     }
   }
   for (Slot& sl : s->slots)
@@ -238,24 +295,40 @@ void bpc_server_default_config(bpc_server_config* c) {
   c->max_inflight = 2;
   c->max_batch_images = 64;
   c->max_batch_codeblocks = 1 << 18;
-  c->hbm_budget_bytes = (size_t)32 << 30;
+  c->hbm_budget_bytes = 0;                 // no output budget: free device memory decides
+  c->hbm_reserve_bytes = (size_t)4 << 30;  // left free for the batch workspaces and other users
+  c->pool_limit_bytes = (size_t)16 << 30;
   c->waves_per_block = 4;
   c->vgpr_cap_waves = 8;  // fastest register budget of the batched kernel on MI355X
+  c->mem_info = nullptr;
+  c->mem_info_ctx = nullptr;
 }
 
 int bpc_server_create(bpc_server** out, const bpc_server_config* cfg) {
-  if (!out) return BPC_ERR_ARG;
+  if (!out) return bpc::fail(BPC_ERR_ARG, "null server");
   bpc_server_config c;
   bpc_server_default_config(&c);
   if (cfg) c = *cfg;
-  if (c.max_inflight < 1 || c.max_inflight > 8 || c.max_batch_images < 1 || c.max_batch_images > 4096 ||
-      c.max_batch_codeblocks < 1)
-    return BPC_ERR_ARG;
+  if (c.max_inflight < 1 || c.max_inflight > 8 || c.max_batch_images < 1 ||
+      c.max_batch_images > bpc::MAX_BATCH_IMAGES || c.max_batch_codeblocks < 1 ||
+      c.max_batch_codeblocks > ((long long)1 << 32))
+    return bpc::fail(BPC_ERR_ARG, "bad server configuration");
   bpc_server* s = new bpc_server();
   s->cfg = c;
+  s->rule.budget = c.hbm_budget_bytes;
+  s->rule.reserve = c.hbm_reserve_bytes;
   if (hipGetDevice(&s->device) != hipSuccess) {
     delete s;
-    return BPC_ERR_NODEVICE;
+    return bpc::fail(BPC_ERR_NODEVICE, "no HIP device");
+  }
+  {
+    size_t fr = 0, tot = 0;
+    const int rc = c.mem_info ? c.mem_info(c.mem_info_ctx, &fr, &tot) : (int)hipMemGetInfo(&fr, &tot);
+    if (rc != 0) {
+      delete s;
+      return bpc::fail(BPC_ERR_HIP, "server: device memory query failed");
+    }
+    s->total_mem = tot;
   }
   s->slots.resize(c.max_inflight);
   for (Slot& sl : s->slots) {
@@ -277,26 +350,35 @@ int bpc_server_create(bpc_server** out, const bpc_server_config* cfg) {
   return BPC_OK;
 }
 
+// A request is checked whole here, so that a malformed one fails alone instead of
+// failing the batch it would join or blocking the queue.
 int bpc_server_submit(bpc_server* s, const bpc_request* r, uint64_t* id) {
-  if (!s || !r || !id) return BPC_ERR_ARG;
-  if (r->direction != BPC_CODE && r->direction != BPC_DECODE) return BPC_ERR_ARG;
-  if (r->size_x <= 0 || r->size_y <= 0 || r->size_x % 64 || r->params.cblock_length < 2 ||
-      r->size_y % r->params.cblock_length)
-    return BPC_ERR_ARG;
+  if (!s || !r || !id) return bpc::fail(BPC_ERR_ARG, "null argument");
+  if (r->direction != BPC_CODE && r->direction != BPC_DECODE) return bpc::fail(BPC_ERR_ARG, "bad direction");
+  const long long ncb = bpc::batch_image_codeblocks(r->size_x, r->size_y, &r->params);
+  if (ncb < 0) return (int)ncb;
+  if (ncb > s->cfg.max_batch_codeblocks)
+    return bpc::fail(BPC_ERR_ARG, "request of %lld codeblocks exceeds the server's batch (%lld)", ncb,
+                     s->cfg.max_batch_codeblocks);
+  if (!r->src || !r->lut) return bpc::fail(BPC_ERR_ARG, "null buffer");
+  if ((uintptr_t)r->src & 7) return bpc::fail(BPC_ERR_ARG, "input must be 8-byte aligned");
+  if (r->lut_len < bpc_lut_len(&r->params))
+    return bpc::fail(BPC_ERR_ARG, "lut_len %d < %d required", r->lut_len, bpc_lut_len(&r->params));
+  const size_t out_bytes = r->direction == BPC_CODE ? (size_t)bpc_stream_words(r->size_x, r->size_y, &r->params) * 4
+                                                    : (size_t)r->size_x * r->size_y * 4;
+  if (!s->rule.possible(out_bytes, s->total_mem))
+    return bpc::fail(BPC_ERR_ARG, "output of %zu bytes can never be admitted (budget %zu, reserve %zu, device %zu)",
+                     out_bytes, s->rule.budget, s->rule.reserve, s->total_mem);
   Req* q = new Req();
   q->r = *r;
-  q->out_bytes = r->direction == BPC_CODE ? (size_t)bpc_stream_words(r->size_x, r->size_y, &r->params) * 4
-                                          : (size_t)r->size_x * r->size_y * 4;
-  if (q->out_bytes > s->cfg.hbm_budget_bytes) {
-    delete q;
-    return BPC_ERR_ARG;
-  }
+  q->out_bytes = out_bytes;
   q->t_submit = Clock::now();
   {
     std::lock_guard<std::mutex> lk(s->mu);
     q->id = s->next_id++;
     s->all[q->id] = q;
     s->pending.push_back(q);
+    s->changes++;
   }
   s->cv_work.notify_one();
   *id = q->id;
@@ -304,14 +386,14 @@ int bpc_server_submit(bpc_server* s, const bpc_request* r, uint64_t* id) {
 }
 
 int bpc_server_wait(bpc_server* s, uint64_t id, int timeout_ms, bpc_result* res) {
-  if (!s || !res) return BPC_ERR_ARG;
+  if (!s || !res) return bpc::fail(BPC_ERR_ARG, "null argument");
   std::unique_lock<std::mutex> lk(s->mu);
   auto it = s->all.find(id);
-  if (it == s->all.end()) return BPC_ERR_ARG;
+  if (it == s->all.end()) return bpc::fail(BPC_ERR_ARG, "unknown request %llu", (unsigned long long)id);
   Req* q = it->second;
   if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms < 0 ? 1 << 30 : timeout_ms),
                            [&] { return q->done; }))
-    return BPC_ERR_TIMEOUT;
+    return bpc::fail(BPC_ERR_TIMEOUT, "request %llu not finished in %d ms", (unsigned long long)id, timeout_ms);
   res->status = q->status;
   res->output = q->out;
   res->ncw = q->ncw;
@@ -321,17 +403,18 @@ int bpc_server_wait(bpc_server* s, uint64_t id, int timeout_ms, bpc_result* res)
 }
 
 int bpc_server_release(bpc_server* s, uint64_t id) {
-  if (!s) return BPC_ERR_ARG;
+  if (!s) return bpc::fail(BPC_ERR_ARG, "null server");
   {
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->all.find(id);
-    if (it == s->all.end() || !it->second->done) return BPC_ERR_ARG;
+    if (it == s->all.end() || !it->second->done) return bpc::fail(BPC_ERR_ARG, "request %llu unknown or not finished", (unsigned long long)id);
     Req* q = it->second;
     s->all.erase(it);
     if (q->admitted) s->committed -= q->out_bytes;
     pool_put(s, q->out, q->out_bytes);
     if (q->ncw) pool_put(s, q->ncw, ncw_bytes(q->r));
     delete q;
+    s->changes++;
   }
   s->cv_work.notify_one();
   return BPC_OK;

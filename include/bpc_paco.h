@@ -202,7 +202,8 @@ typedef struct bpc_image {
 } bpc_image;
 
 typedef struct bpc_batch bpc_batch;
-/* Device work-queue buffers for up to max_images images / max_codeblocks codeblocks. */
+/* Device work-queue buffers for up to max_images (<= 4096) images / max_codeblocks
+ * codeblocks; an image in a batch holds at most 2^20 - 1 codeblocks. */
 int bpc_batch_create(bpc_batch** batch, int max_images, long long max_codeblocks);
 int bpc_batch_destroy(bpc_batch* batch);
 /* Launch shape of the persistent kernel: waves per workgroup (1, 2 or 4, i.e. 2, 4
@@ -214,28 +215,43 @@ int bpc_batch_set_tuning(bpc_batch* batch, int waves_per_block, int waves_per_si
  * kernel keeps the compiler's budget).  The results are the same. */
 int bpc_batch_set_vgpr_cap(bpc_batch* batch, int waves_per_simd);
 /* Encodes (BPC_CODE) or decodes (BPC_DECODE) all images, asynchronously on stream.
- * use_graph != 0 captures the launch sequence into a hipGraph the first time a batch
- * signature (sizes and parameters of every image, tuning) is seen and replays it
- * afterwards.  Errors land in d_status (bpc_status_take). */
+ * Codeblocks whose MSB needs more bitplanes than the images' lut_n_bitplanes
+ * provide for are coded by a second, 28-plane pass, so every legal codeblock is
+ * coded.  use_graph != 0 captures the launch sequence into a hipGraph the first
+ * time a batch signature (geometry and parameters of every image, codeblock
+ * counts, tuning) is seen and replays it afterwards; at most 32 graphs are kept
+ * per batch object, the least recently used is dropped.  Buffer pointers, LUTs
+ * and d_status are uploaded per run, so they may change between replays.
+ * Errors land in d_status (bpc_status_take). */
 int bpc_batch_run(bpc_batch* batch, int direction, const bpc_image* images, int n_images, int use_graph,
                   uint32_t* d_status, void* stream);
+/* Number of captured graphs the batch object holds. */
+int bpc_batch_graph_count(bpc_batch* batch);
 
 /* ---- continuous-batching server ------------------------------------------------
  * Encode / decode requests on device-resident inputs are queued on arrival.  A
- * scheduler thread admits them in arrival order while the HBM budget allows their
- * output buffers, groups admitted requests of one direction into a batch (the
- * batched codec above) and launches it on one of max_inflight streams, so
- * requests that arrive while others are on the GPU start as soon as a slot and
- * memory free up and their work overlaps on the device.  Outputs are allocated by
- * the server and stay in HBM until bpc_server_release.  No counterpart in the
- * reference (one synchronous image per CUDA_BPC call). */
+ * scheduler thread admits them in arrival order while the device's free memory
+ * (less hbm_reserve_bytes) and the optional output budget allow their output
+ * buffers, groups admitted requests of one direction into a batch (the batched
+ * codec above) and launches it on one of max_inflight streams, so requests that
+ * arrive while others are on the GPU start as soon as a slot and memory free up
+ * and their work overlaps on the device.  Outputs are allocated by the server and
+ * stay in HBM until bpc_server_release.  No counterpart in the reference (one
+ * synchronous image per CUDA_BPC call, which allocates its buffers per call,
+ * BPC_PaCo.cu:1277-1289). */
+/* Device memory probe: fills free and total bytes, returns 0 on success. */
+typedef int (*bpc_mem_info_fn)(void* ctx, size_t* free_bytes, size_t* total_bytes);
 typedef struct bpc_server_config {
   int32_t max_inflight;         /* batches on the device at once (streams), 1..8 */
-  int32_t max_batch_images;     /* images per batch */
-  long long max_batch_codeblocks;
-  size_t hbm_budget_bytes;      /* output bytes held by admitted, unreleased requests */
+  int32_t max_batch_images;     /* images per batch, 1..4096 */
+  long long max_batch_codeblocks; /* codeblocks per batch; a larger request is refused at submit */
+  size_t hbm_budget_bytes;      /* cap on output bytes held by admitted, unreleased requests; 0 = none */
+  size_t hbm_reserve_bytes;     /* device memory admission leaves free (batch workspaces, other users) */
+  size_t pool_limit_bytes;      /* released output buffers kept for reuse, at most this many bytes */
   int32_t waves_per_block;      /* tuning of the batched kernel (1, 2 or 4) */
   int32_t vgpr_cap_waves;       /* its register budget: 0 or 5..8 (bpc_batch_set_vgpr_cap) */
+  bpc_mem_info_fn mem_info;     /* NULL: hipMemGetInfo; a caller may inject its own figure */
+  void* mem_info_ctx;
 } bpc_server_config;
 
 typedef struct bpc_request {
@@ -263,6 +279,9 @@ typedef struct bpc_server_stats_t {
 typedef struct bpc_server bpc_server;
 void bpc_server_default_config(bpc_server_config* cfg);
 int bpc_server_create(bpc_server** server, const bpc_server_config* cfg);
+/* Refuses (BPC_ERR_ARG) a request that could never run: bad parameters or geometry,
+ * more codeblocks than max_batch_codeblocks, a short LUT, or an output larger than
+ * the budget or the device less the reserve. */
 int bpc_server_submit(bpc_server* server, const bpc_request* req, uint64_t* id);
 /* Blocks until request id finished (timeout_ms < 0: no limit); fills res. */
 int bpc_server_wait(bpc_server* server, uint64_t id, int timeout_ms, bpc_result* res);

@@ -162,16 +162,48 @@ typedef struct {
   int cpoff;                                  /* cp_sig/cp_sign offset (BPC_PaCo.cu:988-989) */
   uint32_t *n0, *n; /* symbol-count mode (non-NULL): per LUT entry, zeros and all symbols */
   int count_nbp;    /* symbol-count mode: planes >= this are modelled but not counted */
+  /* per-symbol trace (non-NULL): one record per coded symbol, in coding order */
+  orc_trace_rec* trace;
+  int trace_cap, trace_n;
+  int t_plane, t_pass, t_row, t_odd, t_site; /* where the scan is (set by the passes) */
 } orc_cb;
 
+/* One trace record: the printf of BPC_PaCo.cu:266,317,434,445 (THREAD_DEBUG_ID)
+ * kept as data for every lane.  L and S are the interval before the symbol, after
+ * any reservation the symbol made. */
+static void trace_symbol(orc_cb* c, int lane, int idx, int prob, const orc_ac* a, uint32_t sym, int opened) {
+  if (!c->trace) return;
+  if (c->trace_n < c->trace_cap) {
+    orc_trace_rec* r = &c->trace[c->trace_n];
+    r->plane = (int16_t)c->t_plane;
+    r->pass = (int8_t)c->t_pass;
+    r->site = (int8_t)c->t_site;
+    r->row = (int16_t)c->t_row;
+    r->column = (int8_t)(2 * lane + c->t_odd);
+    r->lane = (int8_t)lane;
+    r->lut_index = idx;
+    r->prob = prob;
+    r->L = a->L;
+    r->S = a->S;
+    r->slot = a->slot;
+    r->sym = sym;
+    r->opened = (uint32_t)opened;
+  }
+  c->trace_n++;
+}
+
 /* ---- arithmetic coder, BPC_PaCo.cu:261-295 (encoder) ---- */
-static void ac_encode(orc_cb* c, int lane, uint32_t sym, int prob) {
+static void ac_encode(orc_cb* c, int lane, uint32_t sym, int idx) {
   orc_ac* a = &c->ac[lane];
+  const int prob = c->lut[idx];
+  int opened = 0;
   if (a->S == 0) { /* reservation: BPC_PaCo.cu:268-277 */
     a->L = 0;
     a->S = (1u << 16) - 1;
     a->slot = 1u + c->counter++;
+    opened = 1;
   }
+  trace_symbol(c, lane, idx, prob, a, sym, opened);
   uint32_t aux = ((a->S * (uint32_t)prob) >> 7) + sym; /* MULT_PRECISION = 7 */
   if (sym == 0) {
     a->S = aux;
@@ -197,16 +229,20 @@ static void emit(orc_cb* c, int lane, uint32_t sym, int idx, int bp) {
     }
     return;
   }
-  ac_encode(c, lane, sym, c->lut[idx]);
+  ac_encode(c, lane, sym, idx);
 }
 
 /* ---- arithmetic coder, BPC_PaCo.cu:297-339 (decoder) ---- */
-static uint32_t ac_decode(orc_cb* c, int lane, int prob) {
+static uint32_t ac_decode(orc_cb* c, int lane, int idx) {
   orc_ac* a = &c->ac[lane];
+  const int prob = c->lut[idx];
+  int opened = 0;
   if (a->S == 0) { /* BPC_PaCo.cu:304-319 */
     a->L = 0;
     a->S = (1u << 16) - 1;
     uint32_t slot = 1u + c->counter++;
+    a->slot = slot;
+    opened = 1;
     if (slot < (uint32_t)c->stride) {
       a->cw = (uint32_t)c->rregion[slot];
     } else {
@@ -214,15 +250,19 @@ static uint32_t ac_decode(orc_cb* c, int lane, int prob) {
       c->err = ORC_ERR_OVERFLOW;
     }
   }
+  const orc_ac before = *a;
   uint32_t aux = ((a->S * (uint32_t)prob) >> 7) + 1;
   uint32_t aux2 = a->L + aux;
+  uint32_t sym = 0;
   if (a->cw >= aux2) {
     a->S -= aux;
     a->L = aux2;
-    return 1;
+    sym = 1;
+  } else {
+    a->S = aux - 1;
   }
-  a->S = aux - 1;
-  return 0;
+  trace_symbol(c, lane, idx, prob, &before, sym, opened);
+  return sym;
 }
 
 /* ---- neighbour snapshot of one scan step (BPC_PaCo.cu:633-706, 342-410) ----
@@ -290,11 +330,12 @@ static void sig_step(orc_cb* c, int row, int odd, int bp, int cleanup, int decod
     }
     const int ctx = orc_sig_context(nb[l]);
     uint32_t sym;
+    c->t_site = 0;
     if (!decode) {
       sym = (*w >> (bp + 1)) & 1u;
       emit(c, l, sym, sbase + ctx, bp);
     } else {
-      sym = ac_decode(c, l, c->lut[sbase + ctx]);
+      sym = ac_decode(c, l, sbase + ctx);
     }
     if (cleanup) *w &= ~C_BIT; /* BPC_PaCo.cu:528,563 */
     if (sym) {
@@ -310,10 +351,11 @@ static void sig_step(orc_cb* c, int row, int odd, int bp, int cleanup, int decod
     const uint32_t* n = nb[l];
     const int sctx = orc_sign_context(n[1], n[3], n[4], n[6]); /* (U, L, R, D) */
     const int pidx = c->sign[l] + bp * N_SIGN + (cleanup ? c->cpoff : 0) + (sctx >> 1);
+    c->t_site = 1;
     if (!decode) {
       emit(c, l, ((*w & 1u) == (uint32_t)(sctx & 1)) ? 0u : 1u, pidx, bp);
     } else {
-      uint32_t s = ac_decode(c, l, c->lut[pidx]);
+      uint32_t s = ac_decode(c, l, pidx);
       s = ((s & 1u) == (uint32_t)(sctx & 1)) ? 0u : 1u;
       *w |= s;
     }
@@ -326,11 +368,12 @@ static void ref_step(orc_cb* c, int row, int odd, int bp, int decode, uint32_t m
   for (int l = 0; l < NLANES; ++l) {
     uint32_t* w = &c->w[l][ci];
     const int ridx = c->ref[l] + bp * N_REF;
+    c->t_site = 2;
     if (*w & R_BIT) {
       if (!decode) {
         emit(c, l, (*w >> (bp + 1)) & 1u, ridx, bp);
       } else {
-        uint32_t s = ac_decode(c, l, c->lut[ridx]);
+        uint32_t s = ac_decode(c, l, ridx);
         *w &= ~mask;
         *w |= mask & (((s << 1) + 1u) << bp);
       }
@@ -342,15 +385,25 @@ static void ref_step(orc_cb* c, int row, int odd, int bp, int decode, uint32_t m
 
 /* launchers: rows top to bottom, even column then odd column (BPC_PaCo.cu:633-962) */
 static void sig_pass(orc_cb* c, int bp, int cleanup, int decode, uint32_t mask, int cp3) {
+  c->t_plane = bp;
+  c->t_pass = cleanup ? 2 : 0;
   for (int row = 0; row < c->rows; ++row) {
+    c->t_row = row;
+    c->t_odd = 0;
     sig_step(c, row, 0, bp, cleanup, decode, mask, cp3);
+    c->t_odd = 1;
     sig_step(c, row, 1, bp, cleanup, decode, mask, cp3);
   }
 }
 
 static void ref_pass(orc_cb* c, int bp, int decode, uint32_t mask) {
+  c->t_plane = bp;
+  c->t_pass = 1;
   for (int row = 0; row < c->rows; ++row) {
+    c->t_row = row;
+    c->t_odd = 0;
     ref_step(c, row, 0, bp, decode, mask);
+    c->t_odd = 1;
     ref_step(c, row, 1, bp, decode, mask);
   }
 }
@@ -427,13 +480,17 @@ static int read_coefficients(orc_cb* c, const int32_t* img, int size_x, int cb, 
   return range_err;
 }
 
-int orc_encode_cb(const int32_t* img, int size_x, int size_y, int cb, const int32_t* lut, int lut_len,
-                  const orc_params* p, int32_t* region, int32_t* ncw) {
+static int encode_cb(const int32_t* img, int size_x, int size_y, int cb, const int32_t* lut, int lut_len,
+                     const orc_params* p, int32_t* region, int32_t* ncw, orc_trace_rec* trace, int trace_cap,
+                     int* trace_n) {
   int st = check_params(size_x, size_y, p, lut_len);
   if (st) return st;
   if (cb < 0 || cb >= (size_x / 64) * (size_y / p->cblock_length)) return ORC_ERR_ARG;
   orc_cb* c = (orc_cb*)calloc(1, sizeof(orc_cb));
   if (!c) return ORC_ERR_ARG;
+  c->trace = trace;
+  c->trace_cap = trace_cap;
+  if (trace_n) *trace_n = 0;
   c->stride = stride_of(p);
   c->lut = lut;
   c->wregion = region;
@@ -462,13 +519,27 @@ int orc_encode_cb(const int32_t* img, int size_x, int size_y, int cb, const int3
       c->err = ORC_ERR_OVERFLOW;
   }
   if (ncw) *ncw = (int32_t)c->counter;
+  if (trace_n) *trace_n = c->trace_n;
   st = c->err;
   free(c);
   return st;
 }
 
-int orc_decode_cb(const int32_t* region, int size_x, int size_y, int cb, const int32_t* lut, int lut_len,
-                  const orc_params* p, int32_t* img) {
+int orc_encode_cb(const int32_t* img, int size_x, int size_y, int cb, const int32_t* lut, int lut_len,
+                  const orc_params* p, int32_t* region, int32_t* ncw) {
+  return encode_cb(img, size_x, size_y, cb, lut, lut_len, p, region, ncw, NULL, 0, NULL);
+}
+
+int orc_encode_cb_trace(const int32_t* img, int size_x, int size_y, int cb, const int32_t* lut, int lut_len,
+                        const orc_params* p, int32_t* region, int32_t* ncw, orc_trace_rec* trace, int cap,
+                        int* n) {
+  if (!trace || cap < 0 || !n) return ORC_ERR_ARG;
+  return encode_cb(img, size_x, size_y, cb, lut, lut_len, p, region, ncw, trace, cap, n);
+}
+
+static int decode_cb(const int32_t* region, int size_x, int size_y, int cb, const int32_t* lut, int lut_len,
+                     const orc_params* p, int32_t* img, orc_trace_rec* trace, int trace_cap, int* trace_n) {
+  if (trace_n) *trace_n = 0;
   int st = check_params(size_x, size_y, p, lut_len);
   if (st) return st;
   if (cb < 0 || cb >= (size_x / 64) * (size_y / p->cblock_length)) return ORC_ERR_ARG;
@@ -486,6 +557,8 @@ int orc_decode_cb(const int32_t* region, int size_x, int size_y, int cb, const i
     c->stride = stride_of(p);
     c->lut = lut;
     c->rregion = region;
+    c->trace = trace;
+    c->trace_cap = trace_cap;
     /* Initialize_Coefficients, BPC_PaCo.cu:65-81 */
     for (int l = 0; l < NLANES; ++l)
       for (int k = 0; k < 2 * rows; ++k) c->w[l][k] = p->cpass3 ? C_BIT : 0u;
@@ -494,6 +567,7 @@ int orc_decode_cb(const int32_t* region, int size_x, int size_y, int cb, const i
       run_planes(c, msb, p->cpass3, 1);
       err = c->err;
     }
+    if (trace_n) *trace_n = c->trace_n;
   }
   /* Write_Coefficients, BPC_PaCo.cu:46-63 (zeros for the all-zero sentinel or on error) */
   for (int l = 0; l < NLANES; ++l)
@@ -509,6 +583,17 @@ int orc_decode_cb(const int32_t* region, int size_x, int size_y, int cb, const i
       }
   free(c);
   return err;
+}
+
+int orc_decode_cb(const int32_t* region, int size_x, int size_y, int cb, const int32_t* lut, int lut_len,
+                  const orc_params* p, int32_t* img) {
+  return decode_cb(region, size_x, size_y, cb, lut, lut_len, p, img, NULL, 0, NULL);
+}
+
+int orc_decode_cb_trace(const int32_t* region, int size_x, int size_y, int cb, const int32_t* lut, int lut_len,
+                        const orc_params* p, int32_t* img, orc_trace_rec* trace, int cap, int* n) {
+  if (!trace || cap < 0 || !n) return ORC_ERR_ARG;
+  return decode_cb(region, size_x, size_y, cb, lut, lut_len, p, img, trace, cap, n);
 }
 
 /* ------------------------------------------------------------------------- */

@@ -56,6 +56,33 @@ int orc_encode_cb(const int32_t* img, int size_x, int size_y, int cb, const int3
 int orc_decode_cb(const int32_t* region, int size_x, int size_y, int cb, const int32_t* lut, int lut_len,
                   const orc_params* p, int32_t* img);
 
+/* Per-symbol trace of one codeblock: the reference's debugging printfs
+ * (THREAD_DEBUG_ID, BPC_PaCo.cu:266,317,434,445) kept as one record per coded
+ * symbol of every lane, in coding order (plane, pass, row, column parity, call
+ * site, lane).  L and S are the interval before the symbol, after any slot
+ * reservation that symbol made (opened = 1).  sym is the arithmetic-coded symbol
+ * (for a sign, before the predicted-sign flip).  The region / image is written as
+ * by orc_encode_cb / orc_decode_cb; *n receives the number of symbols (records
+ * beyond cap are counted but not stored). */
+typedef struct orc_trace_rec {
+  int16_t plane;
+  int8_t pass;   /* 0 significance propagation, 1 refinement, 2 cleanup */
+  int8_t site;   /* 0 significance, 1 sign, 2 refinement */
+  int16_t row;
+  int8_t column; /* codeblock column 0..63 (2 * lane + parity) */
+  int8_t lane;
+  int32_t lut_index;
+  int32_t prob;
+  uint32_t L, S;
+  uint32_t slot; /* codeword slot the symbol is coded into */
+  uint32_t sym;
+  uint32_t opened;
+} orc_trace_rec;
+int orc_encode_cb_trace(const int32_t* img, int size_x, int size_y, int cb, const int32_t* lut, int lut_len,
+                        const orc_params* p, int32_t* region, int32_t* ncw, orc_trace_rec* trace, int cap, int* n);
+int orc_decode_cb_trace(const int32_t* region, int size_x, int size_y, int cb, const int32_t* lut, int lut_len,
+                        const orc_params* p, int32_t* img, orc_trace_rec* trace, int cap, int* n);
+
 /* Same as orc_encode/orc_decode restricted to a list of codeblocks (OpenMP over the list). */
 int orc_encode_list(const int32_t* img, int size_x, int size_y, const int32_t* cbs, int ncbs,
                     const int32_t* lut, int lut_len, const orc_params* p, int32_t* stream,

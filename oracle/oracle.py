@@ -62,6 +62,8 @@ def lib():
         L.orc_ac_encode_seq.argtypes = [P, P, i32, P, i32]
         L.orc_ac_decode_seq.argtypes = [P, i32, P, i32, P]
         L.orc_lut_load_text.argtypes = [ctypes.c_char_p, i32, i32, P, i32, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+        L.orc_encode_cb_trace.argtypes = [P, i32, i32, i32, P, i32, PP, P, P, P, i32, ctypes.POINTER(i32)]
+        L.orc_decode_cb_trace.argtypes = [P, i32, i32, i32, P, i32, PP, P, P, i32, ctypes.POINTER(i32)]
         _lib = L
     return _lib
 
@@ -104,6 +106,64 @@ def decode(stream, size_x, size_y, lut, p, nthreads=0):
     img = np.zeros((size_y, size_x), dtype=np.int32)
     st = lib().orc_decode(_ptr(stream), size_x, size_y, _ptr(lut), lut.size, ctypes.byref(p), _ptr(img), nthreads)
     return st, img
+
+
+# orc_trace_rec (bpc_oracle.h): one record per coded symbol
+TRACE_DTYPE = np.dtype([("plane", "<i2"), ("pass", "i1"), ("site", "i1"), ("row", "<i2"), ("column", "i1"),
+                        ("lane", "i1"), ("lut_index", "<i4"), ("prob", "<i4"), ("L", "<u4"), ("S", "<u4"),
+                        ("slot", "<u4"), ("sym", "<u4"), ("opened", "<u4")])
+PASS_NAMES = ("SPP", "MRP", "CP")
+SITE_NAMES = ("significance", "sign", "refinement")
+
+
+def _trace_call(fn, n_hint):
+    cap = n_hint
+    while True:
+        rec = np.zeros(max(cap, 1), dtype=TRACE_DTYPE)
+        n = ctypes.c_int()
+        st = fn(rec, cap, n)
+        if n.value <= cap:
+            return st, rec[: n.value]
+        cap = n.value
+
+
+def encode_cb_trace(img, cb, lut, p):
+    """Encode codeblock cb of a (Y, X) image and return (status, region, ncw, trace):
+    every coded symbol of every lane as a TRACE_DTYPE record, in coding order."""
+    img = np.ascontiguousarray(img, dtype=np.int32)
+    lut = np.ascontiguousarray(lut, dtype=np.int32)
+    Y, X = img.shape
+    region = np.full(stride(p), -1, dtype=np.int32)
+    ncw = np.zeros(1, dtype=np.int32)
+    st, rec = _trace_call(lambda rec, cap, n: lib().orc_encode_cb_trace(
+        _ptr(img), X, Y, cb, _ptr(lut), lut.size, ctypes.byref(p), _ptr(region), _ptr(ncw), _ptr(rec), cap,
+        ctypes.byref(n)), 4096)
+    return st, region, int(ncw[0]), rec
+
+
+def decode_cb_trace(region, size_x, size_y, cb, lut, p):
+    """Decode one codeblock region; returns (status, image with that codeblock, trace)."""
+    region = np.ascontiguousarray(region, dtype=np.int32)
+    lut = np.ascontiguousarray(lut, dtype=np.int32)
+    img = np.zeros((size_y, size_x), dtype=np.int32)
+    st, rec = _trace_call(lambda rec, cap, n: lib().orc_decode_cb_trace(
+        _ptr(region), size_x, size_y, cb, _ptr(lut), lut.size, ctypes.byref(p), _ptr(img), _ptr(rec), cap,
+        ctypes.byref(n)), 4096)
+    return st, img, rec
+
+
+def format_trace(rec, lane=None, limit=None):
+    """Text lines of a trace (optionally one lane), one per symbol."""
+    out = []
+    for r in rec:
+        if lane is not None and r["lane"] != lane:
+            continue
+        out.append("bp=%d %s row=%d col=%d lane=%d %s idx=%d P=%d L=%d S=%d slot=%d sym=%d%s" % (
+            r["plane"], PASS_NAMES[r["pass"]], r["row"], r["column"], r["lane"], SITE_NAMES[r["site"]],
+            r["lut_index"], r["prob"], r["L"], r["S"], r["slot"], r["sym"], " open" if r["opened"] else ""))
+        if limit is not None and len(out) >= limit:
+            break
+    return out
 
 
 def encode_list(img, cbs, lut, p, nthreads=0):

@@ -10,14 +10,18 @@ import synth
 pytestmark = pytest.mark.gpu
 
 
-def make_set(bpc, specs, seed):
-    """specs: list of (size, cpass3, nbp, layout, rows); returns per-image dicts with device buffers."""
+def make_set(bpc, specs, seed, mag=None, stride=0, ll_bits=13, detail_exp=9):
+    """specs: list of (size, cpass3, nbp, layout, rows); returns per-image dicts with device buffers.
+    mag: magnitude bitplanes of the data (default min(nbp, 16)); stride: region words (0: reference)."""
     out = []
     for k, (size, cp3, nbp, layout, rows) in enumerate(specs):
         levels = 5 if size >= 1024 else 3
-        img = synth.synth_image(size, size, levels, seed + 2 * k, mag_bits=min(nbp, 16))
-        p = bpc.params(cpass3=cp3, dwt_levels=levels, cblock_length=rows, lut_n_bitplanes=nbp, layout=layout)
-        lut_arr = bpc.train_lut(synth.synth_image(size, size, levels, seed + 2 * k + 1, mag_bits=min(nbp, 16)),
+        mb = mag if mag is not None else min(nbp, 16)
+        gen = dict(mag_bits=mb, ll_bits=min(ll_bits, mb), detail_exp=detail_exp)
+        img = synth.synth_image(size, size, levels, seed + 2 * k, **gen)
+        p = bpc.params(cpass3=cp3, dwt_levels=levels, cblock_length=rows, lut_n_bitplanes=nbp, layout=layout,
+                       cb_stride_words=stride)
+        lut_arr = bpc.train_lut(synth.synth_image(size, size, levels, seed + 2 * k + 1, **gen),
                                 bpc.params(cpass3=cp3, dwt_levels=levels, cblock_length=rows, lut_n_bitplanes=nbp))
         src = img if layout == 0 else synth.to_tiled(img, rows)
         ncb = p.n_codeblocks(size, size)
@@ -59,6 +63,25 @@ def batch_encode_decode(bpc, batch, items, use_graph=False):
     assert bpc.status_take() == 0, bpc.lib().bpc_last_error()
 
 
+def check_oracle(O, items, k=None):
+    """Codestream and codeword counts of every image against the oracle; the
+    reconstruction against the original."""
+    for j, it in enumerate(items):
+        if k is not None and j != k:
+            continue
+        p, img = it["p"], it["img"]
+        op = O.params(cpass3=p.cpass3, dwt_levels=p.dwt_levels, cblock_length=p.cblock_length,
+                      lut_n_bitplanes=p.lut_n_bitplanes, cb_stride_words=p.cb_stride_words)
+        st, ostream, oncw = O.encode(img, it["lut_arr"], op)
+        assert st == 0, j
+        assert np.array_equal(it["d_cs"].download(), ostream), j
+        assert np.array_equal(it["d_ncw"].download(), oncw), j
+        got = it["d_out"].download()
+        got = got.reshape(img.shape) if p.layout == 0 else synth.from_tiled(got, it["size"], it["size"],
+                                                                             p.cblock_length)
+        assert np.array_equal(got, img), j
+
+
 SPECS = [(512, 0, 15, 1, 64), (768, 1, 15, 0, 64), (1024, 1, 16, 1, 64), (512, 1, 15, 0, 64),
          (1024, 0, 14, 0, 64), (768, 0, 15, 1, 64), (192, 1, 15, 0, 64)]
 
@@ -76,15 +99,7 @@ def test_batch_matches_plain_and_oracle(bpc, O, wpb, wps, cap):
             assert np.array_equal(it["d_cs"].download(), cs), k
             assert np.array_equal(it["d_ncw"].download(), ncw), k
             assert np.array_equal(it["d_out"].download(), rec), k
-            img = it["img"]
-            got = it["d_out"].download()
-            got = got.reshape(img.shape) if it["p"].layout == 0 else synth.from_tiled(got, it["size"], it["size"])
-            assert np.array_equal(got, img), k
-            if k < 3:
-                p = it["p"]
-                st, ostream, _ = O.encode(img, it["lut_arr"], O.params(cpass3=p.cpass3, dwt_levels=p.dwt_levels,
-                                                                       lut_n_bitplanes=p.lut_n_bitplanes))
-                assert st == 0 and np.array_equal(it["d_cs"].download(), ostream), k
+        check_oracle(O, items)
     finally:
         batch.destroy()
         free_set(items)
@@ -142,3 +157,128 @@ def test_batch_rejects_mixed_codeblock_lengths(bpc):
     finally:
         batch.destroy()
         free_set(items)
+
+
+@pytest.mark.parametrize("rows,stride", [(32, 0), (16, 0), (2, 512)])
+def test_batch_short_codeblocks_vs_oracle(bpc, O, rows, stride):
+    """Codeblocks of 32, 16 and 2 rows (2 rows with regions large enough for 16-bit
+    data), every image against the oracle, in both layouts."""
+    specs = [(512, 0, 15, 1, rows), (256, 1, 16, 0, rows), (768, 1, 15, 1, rows), (320, 0, 15, 0, rows)]
+    items = make_set(bpc, specs, seed=6000 + rows, stride=stride)
+    batch = bpc.Batch(8, 1 << 16)
+    try:
+        batch_encode_decode(bpc, batch, items)
+        check_oracle(O, items)
+    finally:
+        batch.destroy()
+        free_set(items)
+
+
+@pytest.mark.parametrize("nbp,mag,stride", [(20, 19, 8192), (28, 27, 16384)])
+def test_batch_20_and_28_planes_vs_oracle(bpc, O, nbp, mag, stride):
+    """The 20- and 28-plane kernels in the batch (wide magnitudes, regions larger
+    than the reference's), every image against the oracle, at two register caps."""
+    specs = [(256, 1, nbp, 1, 64), (512, 0, nbp, 0, 64), (320, 1, nbp, 0, 64)]
+    items = make_set(bpc, specs, seed=7000 + nbp, mag=mag, stride=stride, ll_bits=mag, detail_exp=mag - 4)
+    for cap in (0, 8):
+        batch = bpc.Batch(4, 1 << 14)
+        batch.set_vgpr_cap(cap)
+        try:
+            for it in items:
+                it["d_cs"].fill_bytes(0x5A)
+            batch_encode_decode(bpc, batch, items)
+            check_oracle(O, items)
+        finally:
+            batch.destroy()
+    free_set(items)
+
+
+def test_batch_wide_codeblocks_vs_oracle(bpc, O):
+    """Codeblocks whose MSB is beyond the planes the LUTs' bitplane count provides
+    for (17-bit magnitudes, 15 LUT bitplanes; legal: the reference addresses such
+    rows into the next LUT group) are coded by the batch's 28-plane pass, exactly as
+    by the oracle, with no error status."""
+    specs = [(512, 0, 15, 1, 64), (512, 1, 15, 0, 64), (768, 1, 15, 1, 64)]
+    items = make_set(bpc, specs, seed=7500)
+    for it in items:
+        img = it["img"]
+        s = it["size"]
+        # finest-level HH band (LUT group 2: its rows at MSB 16 stay inside the table)
+        img[s // 2 + 5, s // 2 + 9] = -(1 << 16) - 3
+        img[s - 1, s - 64] = (1 << 16) + 77
+        src = img if it["p"].layout == 0 else synth.to_tiled(img)
+        it["d_in"].upload(np.ascontiguousarray(src).ravel())
+    batch = bpc.Batch(4, 1 << 14)
+    try:
+        for graph in (False, True, True):
+            batch_encode_decode(bpc, batch, items, use_graph=graph)
+            check_oracle(O, items)
+    finally:
+        batch.destroy()
+        free_set(items)
+
+
+def test_batch_graph_key_tells_codeblock_lengths_apart(bpc, O):
+    """Two batches with the same image sizes and the same explicit region size but
+    64- and 32-row codeblocks, on a grid held at its cap (1 wave per workgroup, 1
+    wave per SIMD), replayed from graphs: each must come out as the oracle's."""
+    batch = bpc.Batch(2, 1 << 14)
+    batch.set_tuning(1, 1)
+    try:
+        for rows in (64, 32, 64, 32):
+            items = make_set(bpc, [(4096, 1, 15, 1, rows)], seed=8000, stride=4096)
+            try:
+                batch_encode_decode(bpc, batch, items, use_graph=True)
+                check_oracle(O, items)
+            finally:
+                free_set(items)
+        assert batch.graph_count() == 4  # (encode, decode) x (64, 32 rows)
+    finally:
+        batch.destroy()
+
+
+def test_batch_graph_cache_is_bounded(bpc):
+    """At most 32 captured graphs per batch object; the least recently used go."""
+    batch = bpc.Batch(2, 1 << 12)
+    items = make_set(bpc, [(64 * k, 0, 15, 1, 64) for k in range(2, 38)], seed=8500)
+    try:
+        for it in items:
+            batch_encode_decode(bpc, batch, [it], use_graph=True)
+            assert batch.graph_count() <= 32
+        assert batch.graph_count() == 32
+        for it in items:
+            got = synth.from_tiled(it["d_out"].download(), it["size"], it["size"])
+            assert np.array_equal(got, it["img"])
+    finally:
+        batch.destroy()
+        free_set(items)
+
+
+def test_batch_graph_replay_folds_into_the_current_status_word(bpc):
+    """A replayed graph reports into the status word of the current call: a
+    malformed region decoded with a second status word marks that word only."""
+    items = make_set(bpc, [(512, 1, 15, 1, 64)], seed=8700)
+    it = items[0]
+    st1 = bpc.DeviceBuffer(1, np.uint32)
+    st2 = bpc.DeviceBuffer(1, np.uint32)
+    st1.fill_bytes(0)
+    st2.fill_bytes(0)
+    batch = bpc.Batch(2, 1 << 10)
+    try:
+        enc = [bpc.make_image(it["d_in"], it["d_cs"], it["lut"], 512, 512, it["p"], it["d_ncw"])]
+        dec = [bpc.make_image(it["d_cs"], it["d_out"], it["lut"], 512, 512, it["p"])]
+        batch.run(bpc.CODE, enc, use_graph=True, d_status=st1)
+        batch.run(bpc.DECODE, dec, use_graph=True, d_status=st1)
+        assert bpc.status_take(st1) == 0
+        cs = it["d_cs"].download()
+        cs[0] = 40  # MSB word outside [0, 27]
+        it["d_cs"].upload(cs)
+        batch.run(bpc.DECODE, dec, use_graph=True, d_status=st2)
+        assert bpc.status_take(st1) == 0
+        assert bpc.status_take(st2) == bpc.ERR_MALFORMED
+        assert batch.graph_count() == 2
+    finally:
+        batch.destroy()
+        free_set(items)
+        st1.free()
+        st2.free()

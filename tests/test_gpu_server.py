@@ -1,7 +1,8 @@
 """The continuous-batching server: every request's output (encode: codestream
-regions and codeword counts; decode: coefficients) must be identical whether it
-ran alone (per-image launch) or in the mix, with requests arriving while others
-are on the GPU, and with admission throttled by a small HBM budget."""
+regions and codeword counts; decode: coefficients) must equal the oracle's
+whether it ran alone or in the mix, with requests arriving while others are on the
+GPU, and with admission throttled by a small output budget or by the device's
+free memory (an injected figure); malformed requests fail alone at submit."""
 import random
 import threading
 import time
@@ -14,26 +15,36 @@ import synth
 pytestmark = pytest.mark.gpu
 
 
-def make_pool(bpc, specs, seed):
+def make_pool(bpc, O, specs, seed, wide=False):
+    """Request images with their LUTs and the oracle's codestream (which the decode
+    requests replay); wide: a few 17-bit coefficients beyond 15 LUT bitplanes."""
     pool = []
     for k, (size, cp3, nbp) in enumerate(specs):
         img = synth.synth_image(size, size, 5, seed + 2 * k, mag_bits=nbp - 1)
+        if wide:  # finest-level HH codeblocks (LUT group 2), MSB 16
+            img[size // 2 + 3, size // 2 + 70] = (1 << 16) + 9
+            img[size - 2, size - 1] = -(1 << 16) - 1
         p = bpc.params(cpass3=cp3, lut_n_bitplanes=nbp, layout=bpc.LAYOUT_TILED)
         lut_arr = bpc.train_lut(synth.synth_image(size, size, 5, seed + 2 * k + 1, mag_bits=nbp - 1),
                                 bpc.params(cpass3=cp3, lut_n_bitplanes=nbp))
+        st, cs, ncw = O.encode(img, lut_arr, O.params(cpass3=cp3, lut_n_bitplanes=nbp))
+        assert st == 0
         lut = bpc.Lut.from_array(lut_arr)
         d_img = bpc.DeviceBuffer.from_array(synth.to_tiled(img))
-        d_cs = bpc.DeviceBuffer(bpc.stream_words(size, size, p))
-        d_ncw = bpc.DeviceBuffer(p.n_codeblocks(size, size))
-        d_out = bpc.DeviceBuffer(size * size)
-        bpc.encode_device(d_img, d_cs, size, size, lut, p, 1, d_ncw)
-        bpc.decode_device(d_cs, d_out, size, size, lut, p)
-        assert bpc.status_take() == 0
-        pool.append(dict(size=size, p=p, lut=lut, d_img=d_img, d_cs=d_cs, img=img,
-                         cs=d_cs.download(), ncw=d_ncw.download(), rec=d_out.download()))
-        d_ncw.free()
-        d_out.free()
+        d_cs = bpc.DeviceBuffer.from_array(cs)
+        pool.append(dict(size=size, p=p, lut=lut, d_img=d_img, d_cs=d_cs, img=img, cs=cs, ncw=ncw,
+                         rec=synth.to_tiled(img)))
     return pool
+
+
+def check_result(bpc, it, enc, res):
+    assert res.status == 0
+    if enc:
+        assert np.array_equal(bpc.download_ptr(res.output, it["cs"].size), it["cs"])
+        ncb = it["p"].n_codeblocks(it["size"], it["size"])
+        assert np.array_equal(bpc.download_ptr(res.ncw, ncb), it["ncw"])
+    else:
+        assert np.array_equal(bpc.download_ptr(res.output, it["size"] ** 2), it["rec"])
 
 
 def free_pool(pool):
@@ -47,8 +58,8 @@ SPECS = [(512, 0, 15), (1024, 1, 16), (768, 1, 14), (1536, 0, 15), (512, 1, 16)]
 
 
 @pytest.mark.parametrize("inflight,budget_mib", [(2, 4096), (3, 64), (1, 4096)])
-def test_server_results_match_alone(bpc, inflight, budget_mib):
-    pool = make_pool(bpc, SPECS, seed=5000 + inflight)
+def test_server_results_match_oracle(bpc, O, inflight, budget_mib):
+    pool = make_pool(bpc, O, SPECS, seed=5000 + inflight)
     srv = bpc.Server(max_inflight=inflight, max_batch_images=8, hbm_budget_bytes=budget_mib << 20)
     rng = random.Random(7)
     try:
@@ -69,21 +80,94 @@ def test_server_results_match_alone(bpc, inflight, budget_mib):
         lat = []
         for k, (j, enc) in enumerate(plan):
             res = srv.wait(ids[k], timeout_ms=60000)
-            assert res.status == 0, k
-            it = pool[j]
-            if enc:
-                got = bpc.download_ptr(res.output, it["cs"].size)
-                assert np.array_equal(got, it["cs"]), k
-                ncb = it["p"].n_codeblocks(it["size"], it["size"])
-                assert np.array_equal(bpc.download_ptr(res.ncw, ncb), it["ncw"]), k
-            else:
-                got = bpc.download_ptr(res.output, it["size"] ** 2)
-                assert np.array_equal(got, it["rec"]), k
+            check_result(bpc, pool[j], enc, res)
             lat.append(res.latency_us)
             srv.release(ids[k])
         st = srv.stats()
         assert st.completed == len(plan) and st.pending == 0 and st.committed_bytes == 0
         assert min(lat) > 0
+    finally:
+        srv.destroy()
+        free_pool(pool)
+
+
+def test_server_rejects_malformed_requests_alone(bpc, O):
+    """Requests that could never run are refused at submit (they neither fail the
+    batch they would join nor block the queue); a valid request still goes through."""
+    pool = make_pool(bpc, O, [(512, 1, 15)], seed=5900)
+    it = pool[0]
+    srv = bpc.Server(max_inflight=1, max_batch_codeblocks=32)  # 512^2 = 64 codeblocks
+    srv2 = bpc.Server(max_inflight=1)
+    try:
+        with pytest.raises(bpc.BPCError) as e:
+            srv.submit(bpc.CODE, it["d_img"], it["lut"], 512, 512, it["p"])
+        assert e.value.code == bpc.ERR_ARG and "codeblocks" in str(e.value)
+        bad = bpc.params(cpass3=2)
+        with pytest.raises(bpc.BPCError):
+            srv2.submit(bpc.CODE, it["d_img"], it["lut"], 512, 512, bad)
+        with pytest.raises(bpc.BPCError):
+            srv2.submit(bpc.CODE, it["d_img"], it["lut"], 512 + 32, 512, it["p"])
+        short = bpc.Lut(it["lut"].ptr, it["lut"].len - 1)
+        with pytest.raises(bpc.BPCError):
+            srv2.submit(bpc.CODE, it["d_img"], short, 512, 512, it["p"])
+        with pytest.raises(bpc.BPCError):
+            srv2.submit(bpc.CODE, None, it["lut"], 512, 512, it["p"])
+        rid = srv2.submit(bpc.CODE, it["d_img"], it["lut"], 512, 512, it["p"])
+        check_result(bpc, it, True, srv2.wait(rid, timeout_ms=60000))
+        srv2.release(rid)
+    finally:
+        srv.destroy()
+        srv2.destroy()
+        free_pool(pool)
+
+
+def test_server_admission_by_free_memory(bpc, O):
+    """Admission follows the device's free memory, here an injected figure: with
+    room for one output a second request waits (its wait times out), stays waiting
+    after the first is released while the figure says memory is short, and runs as
+    soon as the figure has room again.  Released outputs are not pooled here, so
+    every admission needs free memory."""
+    pool = make_pool(bpc, O, [(1024, 0, 15)], seed=5950)
+    it = pool[0]
+    out_bytes = 1024 * 1024 * 4  # decode output
+    reserve = 64 << 20
+    fake = {"free": reserve + out_bytes}
+    srv = bpc.Server(max_inflight=1, hbm_reserve_bytes=reserve, pool_limit_bytes=0,
+                     mem_info=lambda: (fake["free"], 288 << 30))
+    try:
+        a = srv.submit(bpc.DECODE, it["d_cs"], it["lut"], 1024, 1024, it["p"])
+        check_result(bpc, it, False, srv.wait(a, timeout_ms=60000))
+        fake["free"] = reserve + out_bytes - 1  # the first output holds the memory
+        b = srv.submit(bpc.DECODE, it["d_cs"], it["lut"], 1024, 1024, it["p"])
+        with pytest.raises(bpc.BPCError) as e:
+            srv.wait(b, timeout_ms=300)
+        assert e.value.code == bpc.ERR_TIMEOUT
+        srv.release(a)
+        with pytest.raises(bpc.BPCError):
+            srv.wait(b, timeout_ms=300)
+        assert srv.stats().pending == 1
+        fake["free"] = reserve + out_bytes
+        check_result(bpc, it, False, srv.wait(b, timeout_ms=60000))
+        srv.release(b)
+        st = srv.stats()
+        assert st.completed == 2 and st.pending == 0 and st.committed_bytes == 0
+    finally:
+        srv.destroy()
+        free_pool(pool)
+
+
+def test_server_wide_codeblocks(bpc, O):
+    """17-bit coefficients with 15 LUT bitplanes: the batch's 28-plane pass codes
+    them, so the server returns the oracle's results with no error."""
+    pool = make_pool(bpc, O, [(1024, 0, 15), (1024, 1, 15)], seed=5970, wide=True)
+    srv = bpc.Server(max_inflight=2)
+    try:
+        plan = [(j, enc) for j in range(2) for enc in (True, False)] * 2
+        ids = [srv.submit(bpc.CODE if enc else bpc.DECODE, pool[j]["d_img"] if enc else pool[j]["d_cs"],
+                          pool[j]["lut"], 1024, 1024, pool[j]["p"]) for j, enc in plan]
+        for rid, (j, enc) in zip(ids, plan):
+            check_result(bpc, pool[j], enc, srv.wait(rid, timeout_ms=60000))
+            srv.release(rid)
     finally:
         srv.destroy()
         free_pool(pool)
