@@ -152,6 +152,7 @@ def lib():
             "bpc_encode_device_i16": (i32, [P, P, i32, i32, P, i32, PP, i32, P, P, P]),
             "bpc_decode_device_i16": (i32, [P, P, i32, i32, P, i32, PP, P, P]),
             "bpc_status_take": (i32, [P, P]),
+            "bpc_codec_device_noalloc": (i32, [i32, P, P, i32, i32, P, i32, PP, i32, P, P, P]),
             "bpc_dev_malloc": (i32, [ctypes.POINTER(P), sz]),
             "bpc_dev_free": (i32, [P]),
             "bpc_dev_memset": (i32, [P, i32, sz, P]),
@@ -462,6 +463,14 @@ def decode_device_i16(d_stream, d_coeffs, size_x, size_y, lut, p, d_status=None,
     """decode_device into int16 coefficients (saturated; out-of-range values are reported)."""
     _check(lib().bpc_decode_device_i16(_dptr(d_stream), _dptr(d_coeffs), size_x, size_y, lut.ptr, lut.len,
                                        ctypes.byref(p), _dptr(d_status), _sp(stream)))
+
+
+def codec_device_noalloc(direction, d_src, d_dst, size_x, size_y, lut, p, d_status, fill_unused=1, d_ncw=None,
+                         stream=None):
+    """encode_device / decode_device without any device allocation (fixed per-device
+    scratch; d_status required; calls overlapping on one GPU race)."""
+    _check(lib().bpc_codec_device_noalloc(direction, _dptr(d_src), _dptr(d_dst), size_x, size_y, lut.ptr, lut.len,
+                                          ctypes.byref(p), fill_unused, _dptr(d_ncw), _dptr(d_status), _sp(stream)))
 
 
 def status_take(d_status=None, stream=None):

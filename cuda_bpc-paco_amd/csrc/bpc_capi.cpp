@@ -19,6 +19,8 @@
 #include "../../include/bpc_paco.h"
 #include "bpc_internal.h"
 
+static_assert(bpc::NOALLOC_SCRATCH_BYTES == BPC_NOALLOC_SCRATCH_BYTES, "fixed scratch size in the header");
+
 using bpc::check_image;
 using bpc::check_params;
 using bpc::fail;
@@ -125,9 +127,11 @@ int scratch_release(int dev, hipStream_t stream) {
   return BPC_OK;
 }
 
+// fixed_scratch (nullable): bitplane slots of fixed_bytes to use instead of the
+// library's scratch; the grid is then limited to the waves whose slots fit.
 int launch(int decode, const int32_t* src, int32_t* dst, int size_x, int size_y, const int32_t* d_lut, int lut_len,
            const bpc_params* p, int fill, int32_t* d_ncw, uint32_t* d_status, hipStream_t stream, int maxp,
-           int in16 = 0) {
+           int in16 = 0, void* fixed_scratch = nullptr, size_t fixed_bytes = 0) {
   bpc::KArgs a;
   a.src = src;
   a.dst = dst;
@@ -155,7 +159,16 @@ int launch(int decode, const int32_t* src, int32_t* dst, int size_x, int size_y,
   // instead of two in lock step (the launch time is the longest chain).
   a.single = (long long)a.ncb <= 4LL * device_cus(dev) ? 1 : 0;
   const long long units = a.single ? a.ncb : (a.ncb + 1) / 2;
-  const int grid = (int)std::min<long long>(units, (long long)per_cu * device_cus(dev));
+  int grid = (int)std::min<long long>(units, (long long)per_cu * device_cus(dev));
+  if (fixed_scratch) {
+    const size_t slots = fixed_bytes / bpc::scratch_bytes_per_wave(maxp);
+    if (slots < 1) return fail(BPC_ERR_ARG, "fixed scratch of %zu bytes holds no %d-plane slot", fixed_bytes, maxp);
+    grid = (int)std::min<size_t>((size_t)grid, slots);  // the waves grid-stride over the rest
+    a.scratch = fixed_scratch;
+    hipError_t e = bpc::launch_codec(decode, p->cpass3, maxp, a, grid, stream);
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    return BPC_OK;
+  }
   const size_t need = (size_t)grid * bpc::scratch_bytes_per_wave(maxp);
   std::lock_guard<std::mutex> lk(g_scratch_mu);
   int st = scratch_acquire(dev, need, stream, &a.scratch);
@@ -235,6 +248,25 @@ int bpc_decode_device_i16(const int32_t* d_stream, int16_t* d_coeffs, int size_x
   }
   return launch(1, d_stream, reinterpret_cast<int32_t*>(d_coeffs), size_x, size_y, d_lut, lut_len, p, 0, nullptr,
                 d_status, (hipStream_t)stream, bpc::planes_for(p->lut_n_bitplanes), 1);
+}
+
+int bpc_codec_device_noalloc(int direction, const int32_t* d_src, int32_t* d_dst, int size_x, int size_y,
+                             const int32_t* d_lut, int lut_len, const bpc_params* p, int fill_unused,
+                             int32_t* d_ncw, uint32_t* d_status, void* stream) {
+  if (direction != BPC_CODE && direction != BPC_DECODE) return fail(BPC_ERR_ARG, "direction must be CODE or DECODE");
+  int st = check_device_call(d_src, d_dst, size_x, size_y, d_lut, lut_len, p);
+  if (st) return st;
+  if (!d_status) return fail(BPC_ERR_ARG, "d_status is required (this call allocates no status word)");
+  const int maxp = bpc::planes_for(p->lut_n_bitplanes);
+  if (bpc::scratch_bytes_per_wave(maxp) > bpc::NOALLOC_SCRATCH_BYTES)
+    return fail(BPC_ERR_ARG, "a %d-plane slot of %zu bytes exceeds the fixed scratch of %zu bytes", maxp,
+                bpc::scratch_bytes_per_wave(maxp), bpc::NOALLOC_SCRATCH_BYTES);
+  void* scr = nullptr;
+  hipError_t e = bpc::noalloc_scratch(&scr);
+  if (e != hipSuccess) return hip_fail(e, "fixed scratch address");
+  return launch(direction == BPC_DECODE, d_src, d_dst, size_x, size_y, d_lut, lut_len, p,
+                direction == BPC_CODE ? fill_unused : 0, direction == BPC_CODE ? d_ncw : nullptr, d_status,
+                (hipStream_t)stream, maxp, 0, scr, bpc::NOALLOC_SCRATCH_BYTES);
 }
 
 int bpc_status_take(uint32_t* d_status, void* stream) {

@@ -96,6 +96,12 @@ struct TrainArgs {
 constexpr int TRAIN_MAXP = 28;   // every bitplane a coefficient below 2^28 can have
 constexpr int TRAIN_WAVES = 8;   // waves per trainer workgroup (one shared LDS histogram)
 
+// Bitplane scratch of the entry point that allocates nothing (bpc_codec_device_noalloc):
+// a fixed array per device; its launches use at most this many bytes of slots.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)32 << 20;
+// Device address of that array on the current device (asked once per device).
+hipError_t noalloc_scratch(void** ptr);
+
 // Kernel variants hold this many bitplanes in registers.
 int planes_for(int nbp);
 // Bitplane scratch of one wave: 2 row halves x MAXP planes x 64 lanes x 8 bytes.

@@ -22,6 +22,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "bpc_internal.h"
 
@@ -1146,6 +1147,32 @@ hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t str
   const size_t blocks = std::min<size_t>((n16 + 255) / 256, 256 * 64);
   hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, (const int4*)src, (int4*)dst, n16);
   return hipGetLastError();
+}
+
+// ---- fixed bitplane scratch of the entry point that allocates nothing ---------
+// A file-scope device array: every device that loads this code object holds its own
+// copy, at its own address, until the process ends.  In host code the array's name
+// is only a stand-in for the symbol; its device address comes from
+// hipGetSymbolAddress, asked once per device with that device current.
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+hipError_t noalloc_scratch(void** ptr) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static void* addr[kMaxDevices] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[dev]) {
+    void* a = nullptr;
+    e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[dev] = a;
+  }
+  *ptr = addr[dev];
+  return hipSuccess;
 }
 
 int planes_for(int nbp) {

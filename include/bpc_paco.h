@@ -130,6 +130,23 @@ int bpc_encode_device_i16(const int16_t* d_coeffs, int32_t* d_stream, int size_x
                           void* stream);
 int bpc_decode_device_i16(const int32_t* d_stream, int16_t* d_coeffs, int size_x, int size_y, const int32_t* d_lut,
                           int lut_len, const bpc_params* p, uint32_t* d_status, void* stream);
+/* No counterpart in the reference: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  bpc_encode_device (direction BPC_CODE) or bpc_decode_device
+ * (BPC_DECODE, where fill_unused and d_ncw are ignored) with the same results, but
+ * no call to hipMalloc, hipFree or any allocating routine: the bitplane scratch is a
+ * fixed array of BPC_NOALLOC_SCRATCH_BYTES per device (held until the process
+ * ends), and the launch uses at most as many waves as it has slots for (16 KiB per
+ * wave for up to 16 LUT bitplanes, 20 KiB up to 20, 28 KiB beyond; any image size
+ * is accepted, larger ones take more codeblocks per wave).  d_status is required.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU (on
+ * different streams, or from different threads) race and must be ordered by the
+ * caller.  No part of the array needs clearing: each launch writes every slot word
+ * it reads. */
+#define BPC_NOALLOC_SCRATCH_BYTES ((size_t)32 << 20)
+int bpc_codec_device_noalloc(int direction, const int32_t* d_src, int32_t* d_dst, int size_x, int size_y,
+                             const int32_t* d_lut, int lut_len, const bpc_params* p, int fill_unused,
+                             int32_t* d_ncw, uint32_t* d_status, void* stream);
 /* Synchronizes stream, maps the status bits to a BPC_* code and clears them. */
 int bpc_status_take(uint32_t* d_status, void* stream);
 
