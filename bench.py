@@ -5,13 +5,20 @@ and LUTs resident in HBM.  The default workload is the plain per-image launch on
 4096x4096 image (4096 codeblocks of 64x64, 5-level DWT, 16 magnitude bitplanes,
 3 coding passes), synthetic data drawn as the workload definition states.
 
-Prints one JSON line (metric, value in Msamples/s, roofline of the dominant kernel,
-CPU baseline from the oracle on the host cores).
+Prints one JSON line (metric, value in Msamples/s, roofline of the dominant kernel
+with its HBM traffic measured by two rocprofv3 counter passes over the same
+kernels, CPU baseline from the oracle on the host cores) whose last object,
+`summary`, holds the headline ratios.  The per-setting tables (autotuner sweep,
+served-rate sweep) go to a detail JSON file (--detail-out).
 """
 import argparse
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -23,6 +30,7 @@ import bpcpaco as bpc  # noqa: E402
 import synth  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+DETAIL = {}  # per-setting tables, written to --detail-out
 METRIC = "encode+decode Msamples/s (4096² 5-lvl DWT), batched/served vs plain; p50/p99"
 
 
@@ -39,27 +47,114 @@ def make_workload(size, cp3, nbp, mag_bits, ll_bits, detail_exp, seed, levels=5)
     return img, lut, p
 
 
-def measured_traffic(workload, kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary
-    of this workload (FETCH_SIZE/WRITE_SIZE scaled by a plain copy), when it was
-    taken on the current kernel source; otherwise None."""
-    import hashlib
-    path = os.path.join(ROOT, "profiles", "bpc_codec_%s.traffic.json" % workload)
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        rec = json.load(f)
-    with open(os.path.join(ROOT, "cuda_bpc-paco_amd", "csrc", "bpc_kernels.hip"), "rb") as f:
-        sha = hashlib.sha256(f.read()).hexdigest()
-    if rec.get("kernel_source_sha256") != sha:
-        return None
-    return rec["traffic_bytes_per_launch"].get(kernel)
+KERNEL_KEYS = {"encoder": "bpc_codec_kernel<0,", "decoder": "bpc_codec_kernel<1,", "copy": "copy_kernel"}
+CALIBRATION_MIB = 256
+
+
+def traffic_child(args):
+    """Child run under rocprofv3 --pmc: two encode+decode steps of the plain
+    workload (data saved by the parent) and two plain copies of 256 MiB (the
+    counters' calibration).  Prints nothing on success."""
+    d = np.load(args.traffic_child)
+    img, lut_arr = d["img"], d["lut"]
+    size = img.shape[0]
+    p = bpc.params(cpass3=int(d["cp3"]), lut_n_bitplanes=int(d["nbp"]), layout=bpc.LAYOUT_TILED)
+    lut = bpc.Lut.from_array(lut_arr)
+    d_in = bpc.DeviceBuffer.from_array(synth.to_tiled(img))
+    d_cs = bpc.DeviceBuffer(bpc.stream_words(size, size, p))
+    d_out = bpc.DeviceBuffer(size * size)
+    d_ncw = bpc.DeviceBuffer(p.n_codeblocks(size, size))
+    for _ in range(2):
+        bpc.encode_device(d_in, d_cs, size, size, lut, p, 1, d_ncw)
+        bpc.decode_device(d_cs, d_out, size, size, lut, p)
+    if bpc.status_take() != 0:
+        raise RuntimeError("traffic child: codec status")
+    n = CALIBRATION_MIB << 20
+    a, b = bpc.DeviceBuffer(n // 4), bpc.DeviceBuffer(n // 4)
+    a.fill_bytes(1)
+    for _ in range(2):
+        b.copy_from(a)
+    bpc.device_sync()
+
+
+def counter_medians(outdir, counter):
+    """Median counter value per kernel (encoder, decoder, copy) from a rocprofv3
+    counter-collection CSV."""
+    import csv
+    vals = {k: [] for k in KERNEL_KEYS}
+    files = glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise RuntimeError("no counter collection output in %s" % outdir)
+    for path in files:
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name") != counter:
+                    continue
+                for k, key in KERNEL_KEYS.items():
+                    if key in row.get("Kernel_Name", ""):
+                        vals[k].append(float(row["Counter_Value"]))
+    if not all(vals.values()):
+        raise RuntimeError("counter %s missing for %s" % (counter, [k for k, v in vals.items() if not v]))
+    return {k: float(np.median(v)) for k, v in vals.items()}
+
+
+def measure_traffic(img, lut_arr, cp3, nbp):
+    """HBM bytes per launch of the encoder and the decoder: FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch) from two rocprofv3 --pmc runs of a child process,
+    each scaled by what the counter reads for a plain copy of known size."""
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return None, "rocprofv3 not found"
+    tmp = tempfile.mkdtemp(prefix="bpc_traffic_")
+    try:
+        data = os.path.join(tmp, "plain.npz")
+        np.savez(data, img=img, lut=lut_arr, cp3=cp3, nbp=nbp)
+        med = {}
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            out = os.path.join(tmp, counter)
+            cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--output-format", "csv", "-d", out,
+                   "-o", "run", "--", sys.executable, os.path.abspath(__file__), "--traffic-child", data]
+            log("traffic: %s pass" % counter)
+            r = subprocess.run(cmd, capture_output=True, text=True, cwd=tmp)
+            if r.returncode != 0:
+                return None, "rocprofv3 %s pass exited %d: %s" % (counter, r.returncode, r.stderr[-300:])
+            med[counter] = counter_medians(out, counter)
+        cal = float(CALIBRATION_MIB << 20)
+        f_scale = cal / (med["FETCH_SIZE"]["copy"] * 1024.0)
+        w_scale = cal / (med["WRITE_SIZE"]["copy"] * 1024.0)
+        res = {"fetch_scale": round(f_scale, 4), "write_scale": round(w_scale, 4)}
+        for k in ("encoder", "decoder"):
+            res[k] = {"fetch_kib": med["FETCH_SIZE"][k], "write_kib": med["WRITE_SIZE"][k],
+                      "hbm_bytes": int(round(1024.0 * (med["FETCH_SIZE"][k] * f_scale +
+                                                       med["WRITE_SIZE"][k] * w_scale)))}
+        return res, None
+    except Exception as e:  # the line is still printed, with the reason
+        return None, "traffic measurement failed: %s" % e
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(img, lut_arr, p, cp3, nbp, budget_s=20.0):
-    """Oracle (C, OpenMP over codeblocks) encode+decode of the same image on the host cores."""
+    """Oracle (C, OpenMP over codeblocks, built -O3 -march=native on this host)
+    encode+decode of the same image on the host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
+    try:
+        flags = O.use_native()
+    except Exception as e:  # no compiler here: the portable build
+        log("native oracle build failed (%s); portable build" % e)
+        flags = "-O3 -march=x86-64-v2 -fopenmp"
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     op = O.params(cpass3=cp3, dwt_levels=p.dwt_levels, lut_n_bitplanes=nbp)
     Y, X = img.shape
@@ -77,6 +172,7 @@ def cpu_baseline(img, lut_arr, p, cp3, nbp, budget_s=20.0):
         reps += 1
         enc_stream = stream
     return {"value": samples / total / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "build": flags,
             "sample": "%d x encode+decode of the full %dx%d image (oracle, OpenMP over codeblocks)" % (reps, X, Y)}, \
         enc_stream
 
@@ -212,7 +308,10 @@ def bench_batched(args, check=True):
            "batched_msamples_s": round(vb, 2), "plain_96_launches_msamples_s": round(vp, 2),
            "batched_vs_plain": round(vb / vp, 3),
            "batched_step_ms_p50_p99": percentiles(lat), "plain_step_ms_p50_p99": percentiles(plat),
-           "autotune": tune, "chosen": best, "graph_replay": True}
+           "autotune_settings": len(tune), "chosen": best,
+           "chosen_encode_ms": min(t["encode_ms"] for t in tune), "chosen_decode_ms": min(t["decode_ms"] for t in tune),
+           "graph_replay": True}
+    DETAIL["batched_autotune"] = tune
     batch.destroy()
     for it in items:
         for k in ("d_in", "d_cs", "d_out", "d_ncw"):
@@ -271,14 +370,17 @@ def bench_served(args):
         return sum(pool[j]["size"] ** 2 for j, _ in plan) / wall / 1e6, lat
 
     plain_loop(make_plan(8, 1))
-    n_req = 240
+    n_req = args.served_requests
     plain_rate, plain_lat = plain_loop(make_plan(n_req, 2))
     mean_samples = float(np.mean([it["size"] ** 2 for it in pool]))
     capacity_rps = plain_rate * 1e6 / mean_samples  # one-at-a-time service rate
-    srv = bpc.Server(max_inflight=3, max_batch_images=64, hbm_budget_bytes=24 << 30)
+    srv = bpc.Server(max_inflight=3, max_batch_images=64)
     sweep = []
+    # offered rate as a multiple of the plain loop's capacity: coarse below 2x, fine
+    # between 2x and 3.5x where the server's knee lies
+    mults = (0.5, 1.0, 1.5, 2.0, 2.25, 2.5, 2.75, 3.0, 3.25, 3.5, 4.0)
     try:
-        for mult in (0.5, 1.0, 2.0, 3.0, 4.0):
+        for mult in mults:
             rate = mult * capacity_rps
             plan = make_plan(n_req, int(mult * 100))
             rng = random.Random(int(mult * 1000))
@@ -299,34 +401,53 @@ def bench_served(args):
             th.start()
             t0 = time.perf_counter()
             t_next = t0
-            for k, (j, enc) in enumerate(plan):
-                t_next += gaps[k]
-                dt = t_next - time.perf_counter()
-                if dt > 0:
-                    time.sleep(dt)
-                it = pool[j]
-                ids[k] = srv.submit(bpc.CODE if enc else bpc.DECODE, it["d_img"] if enc else it["d_cs"], it["lut"],
-                                    it["size"], it["size"], it["p"])
-            th.join()
+            try:
+                for k, (j, enc) in enumerate(plan):
+                    t_next += gaps[k]
+                    dt = t_next - time.perf_counter()
+                    if dt > 0:
+                        time.sleep(dt)
+                    it = pool[j]
+                    ids[k] = srv.submit(bpc.CODE if enc else bpc.DECODE, it["d_img"] if enc else it["d_cs"],
+                                        it["lut"], it["size"], it["size"], it["p"])
+            finally:
+                th.join()
             if any(r.status != 0 for r in results):
                 raise RuntimeError("served request failed")
+            t_last_submit = t_next
             wall = max(done_t) - t0
             lat = [r.latency_us / 1e3 for r in results]
             samples = sum(pool[j]["size"] ** 2 for j, _ in plan)
-            sweep.append({"offered_rps": round(rate, 1), "offered_msamples_s": round(rate * mean_samples / 1e6, 1),
+            offered = samples / (t_last_submit - t0) / 1e6
+            sweep.append({"offered_x_plain": mult, "offered_rps": round(rate, 1),
+                          "offered_msamples_s": round(offered, 1),
                           "sustained_msamples_s": round(samples / wall / 1e6, 1),
                           "latency_ms_p50_p99": percentiles(lat), "requests": len(plan)})
+            log("served x%.2f: offered %.0f, sustained %.0f Msamples/s, p99 %.1f ms" % (
+                mult, offered, sweep[-1]["sustained_msamples_s"], sweep[-1]["latency_ms_p50_p99"][1]))
         st = srv.stats()
         batches = int(st.batches)
     finally:
         srv.destroy()
+    DETAIL["served_sweep"] = sweep
+    # knee: the highest offered rate the server keeps up with (sustained >= 95% of offered)
+    kept = [r for r in sweep if r["sustained_msamples_s"] >= 0.95 * r["offered_msamples_s"]]
+    knee = max(kept, key=lambda r: r["offered_msamples_s"]) if kept else min(sweep, key=lambda r: r["offered_rps"])
     best = max(sweep, key=lambda r: r["sustained_msamples_s"])
     out = {"workload": "served (Poisson, 50% encode / 50% decode, sizes {2048,3072,4096,6144}^2, 14-16 bitplanes, "
                        "2/3 passes)",
            "plain_loop_msamples_s": round(plain_rate, 1), "plain_loop_latency_ms_p50_p99": percentiles(plain_lat),
-           "sweep": sweep, "best_sustained_msamples_s": best["sustained_msamples_s"],
-           "best_vs_plain_loop": round(best["sustained_msamples_s"] / plain_rate, 3),
-           "batches_launched": batches, "server": {"max_inflight": 3, "max_batch_images": 64}}
+           "requests_per_rate": n_req, "rates": len(sweep),
+           "knee": {"offered_msamples_s": knee["offered_msamples_s"],
+                    "sustained_msamples_s": knee["sustained_msamples_s"],
+                    "latency_ms_p50_p99": knee["latency_ms_p50_p99"],
+                    "vs_plain_loop": round(knee["sustained_msamples_s"] / plain_rate, 3)},
+           "max_sustained": {"offered_msamples_s": best["offered_msamples_s"],
+                             "sustained_msamples_s": best["sustained_msamples_s"],
+                             "latency_ms_p50_p99": best["latency_ms_p50_p99"],
+                             "vs_plain_loop": round(best["sustained_msamples_s"] / plain_rate, 3)},
+           "batches_launched": batches,
+           "server": {"max_inflight": 3, "max_batch_images": 64, "admission": "device free memory less 4 GiB"}}
     for it in pool:
         it["d_img"].free()
         it["d_cs"].free()
@@ -673,7 +794,15 @@ def main():
                          "workload's autotuner overrides it when both run)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB 5 times (HBM counter calibration)")
+    ap.add_argument("--served-requests", type=int, default=600, help="requests per offered rate")
+    ap.add_argument("--detail-out", default=os.path.join(ROOT, "gpurun_out", "bench_detail.json"),
+                    help="JSON file for the autotuner and served-rate tables ('' = none)")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
+    ap.add_argument("--traffic-child", default="", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.traffic_child:
+        traffic_child(args)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: every codeblock is independent and a multi-GPU run "
               "would be independent replicas; run with --gpus 1", file=sys.stderr)
@@ -763,6 +892,8 @@ def main():
     alg_bytes = float(ncb * 16384 + 4 * (ncb + int(ncw.sum())))
     dom_name, dom_ms = ("decoder", dec_ms) if dec_ms >= enc_ms else ("encoder", enc_ms)
     achieved = alg_bytes / (dom_ms * 1e-3) / 1e9
+    traffic, traffic_err = (None, "skipped (--no-traffic)") if args.no_traffic else \
+        measure_traffic(img, lut_arr, cp3, nbp)
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -787,8 +918,14 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "bpc_codec_kernel (%s)" % dom_name,
                      "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6),
-                     "traffic": measured_traffic("plain_4096_3pass", dom_name),
-                     "algorithmic_bytes_per_launch": alg_bytes},
+                     "traffic": traffic[dom_name]["hbm_bytes"] if traffic else None,
+                     "algorithmic_bytes_per_launch": alg_bytes,
+                     "traffic_by_kernel": ({k: {"hbm_bytes": traffic[k]["hbm_bytes"],
+                                                "vs_algorithmic": round(traffic[k]["hbm_bytes"] / alg_bytes, 3)}
+                                            for k in ("encoder", "decoder")} if traffic else None),
+                     "traffic_source": ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a %d MiB copy "
+                                        "(x%.3f / x%.3f)" % (CALIBRATION_MIB, traffic["fetch_scale"],
+                                                             traffic["write_scale"]) if traffic else traffic_err)},
         "cpu_baseline": None,
         "bit_exact_vs_oracle": None,
     }
@@ -820,6 +957,34 @@ def main():
         out["drop_in"] = bench_drop_in(args)
     if "dwt" in wl:
         out["dwt53"] = bench_dwt(args, check=not args.no_cpu)
+    # headline ratios last, so that the end of the line carries them
+    summary = {"plain_msamples_s": out["value"], "roofline_frac": out["roofline"]["frac"]}
+    if out["roofline"]["traffic_by_kernel"]:
+        summary["traffic_vs_algorithmic"] = {k: v["vs_algorithmic"]
+                                             for k, v in out["roofline"]["traffic_by_kernel"].items()}
+    if out.get("cpu_baseline"):
+        summary["cpu_baseline_msamples_s"] = round(out["cpu_baseline"]["value"], 2)
+    if "batched" in out:
+        summary["batched_vs_plain"] = out["batched"]["batched_vs_plain"]
+        summary["batched_msamples_s"] = out["batched"]["batched_msamples_s"]
+    if "served" in out:
+        k = out["served"]["knee"]
+        summary["served_knee_vs_plain_loop"] = k["vs_plain_loop"]
+        summary["served_knee_msamples_s"] = k["sustained_msamples_s"]
+        summary["served_knee_latency_ms_p50_p99"] = k["latency_ms_p50_p99"]
+        summary["served_max_vs_plain_loop"] = out["served"]["max_sustained"]["vs_plain_loop"]
+    if "large" in out:
+        summary["large_msamples_s"] = out["large"]["msamples_s"]
+    out["summary"] = summary
+    if args.detail_out and DETAIL:
+        try:
+            os.makedirs(os.path.dirname(os.path.abspath(args.detail_out)), exist_ok=True)
+            with open(args.detail_out, "w") as f:
+                json.dump(DETAIL, f, indent=1)
+            out["detail_file"] = os.path.relpath(os.path.abspath(args.detail_out), ROOT)
+        except OSError as e:
+            log("detail tables not written: %s" % e)
+        out["summary"] = out.pop("summary")
     print(json.dumps(out), flush=True)
 
 

@@ -35,6 +35,21 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+NATIVE_FLAGS = "-O3 -march=native -fopenmp"
+
+
+def use_native():
+    """Builds the oracle for the host it runs on (make native: -O3 -march=native
+    -fopenmp) and loads that build instead of the portable one; for the CPU
+    baseline.  Must be called before the first use.  Returns the flags used."""
+    global _LIB_PATH
+    if _lib is not None:
+        raise RuntimeError("oracle already loaded")
+    subprocess.run(["make", "-s", "-C", _HERE, "native"], check=True)
+    _LIB_PATH = os.path.join(_HERE, "_build", "native", "libbpc_oracle.so")
+    return NATIVE_FLAGS
+
+
 def lib():
     global _lib
     if _lib is None:
