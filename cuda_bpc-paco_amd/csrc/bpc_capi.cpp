@@ -73,14 +73,14 @@ int device_cus(int dev) {
 
 // Resident waves per CU of a per-image kernel variant (an occupancy query, asked
 // once per device and variant).
-int resident_waves_cached(int dev, int decode, int cpass3, int maxp) {
+int resident_waves_cached(int dev, int decode, int cpass3, int maxp, int reg_planes) {
   static std::mutex mu;
-  static std::map<std::tuple<int, int, int, int>, int> cache;
-  const auto key = std::make_tuple(dev, decode, cpass3, maxp);
+  static std::map<std::tuple<int, int, int, int, int>, int> cache;
+  const auto key = std::make_tuple(dev, decode, cpass3, maxp, reg_planes);
   std::lock_guard<std::mutex> lk(mu);
   auto it = cache.find(key);
   if (it != cache.end()) return it->second;
-  int per_cu = bpc::codec_resident_waves_per_cu(decode, cpass3, maxp);
+  int per_cu = bpc::codec_resident_waves_per_cu(decode, cpass3, maxp, reg_planes);
   if (per_cu <= 0) per_cu = 1;
   cache[key] = per_cu;
   return per_cu;
@@ -153,19 +153,28 @@ int launch(int decode, const int32_t* src, int32_t* dst, int size_x, int size_y,
   if (a.ncb <= 0) return BPC_OK;
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
-  int per_cu = resident_waves_cached(dev, decode, p->cpass3, maxp);
+  const long long cus = device_cus(dev);
   // An image with no more codeblocks than the device has SIMDs cannot fill it with
   // pairs: one codeblock per wave then runs each codeblock's chain on its own SIMD
   // instead of two in lock step (the launch time is the longest chain).
-  a.single = (long long)a.ncb <= 4LL * device_cus(dev) ? 1 : 0;
+  a.single = (long long)a.ncb <= 4LL * cus ? 1 : 0;
   const long long units = a.single ? a.ncb : (a.ncb + 1) / 2;
-  int grid = (int)std::min<long long>(units, (long long)per_cu * device_cus(dev));
+  // When every wave the image needs is resident at once with the bitplanes held in
+  // registers, that variant runs: no scratch slot, no plane traffic to memory.
+  if (maxp <= 20 && units <= (long long)resident_waves_cached(dev, decode, p->cpass3, maxp, 1) * cus) {
+    a.scratch = nullptr;
+    hipError_t e = bpc::launch_codec(decode, p->cpass3, maxp, 1, a, (int)units, stream);
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    return BPC_OK;
+  }
+  int per_cu = resident_waves_cached(dev, decode, p->cpass3, maxp, 0);
+  int grid = (int)std::min<long long>(units, (long long)per_cu * cus);
   if (fixed_scratch) {
     const size_t slots = fixed_bytes / bpc::scratch_bytes_per_wave(maxp);
     if (slots < 1) return fail(BPC_ERR_ARG, "fixed scratch of %zu bytes holds no %d-plane slot", fixed_bytes, maxp);
     grid = (int)std::min<size_t>((size_t)grid, slots);  // the waves grid-stride over the rest
     a.scratch = fixed_scratch;
-    hipError_t e = bpc::launch_codec(decode, p->cpass3, maxp, a, grid, stream);
+    hipError_t e = bpc::launch_codec(decode, p->cpass3, maxp, 0, a, grid, stream);
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
     return BPC_OK;
   }
@@ -173,7 +182,7 @@ int launch(int decode, const int32_t* src, int32_t* dst, int size_x, int size_y,
   std::lock_guard<std::mutex> lk(g_scratch_mu);
   int st = scratch_acquire(dev, need, stream, &a.scratch);
   if (st) return st;
-  hipError_t e = bpc::launch_codec(decode, p->cpass3, maxp, a, grid, stream);
+  hipError_t e = bpc::launch_codec(decode, p->cpass3, maxp, 0, a, grid, stream);
   if (e != hipSuccess) return hip_fail(e, "kernel launch");
   return scratch_release(dev, stream);
 }

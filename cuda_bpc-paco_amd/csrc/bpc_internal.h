@@ -26,7 +26,8 @@ struct KArgs {
   int fill;            // encoder: write -1 to unused region words
   int single;          // one codeblock per wave (lanes 32-63 idle) instead of a pair
   int in16;            // coefficients are int16 (encoder input / decoder output), else int32
-  void* scratch;       // per wave of the grid: 2 * MAXP * 64 uint2 (scratch_bytes_per_wave)
+  void* scratch;       // per wave of the grid: 2 * MAXP * 64 uint2 (scratch_bytes_per_wave); unused
+                       // by the variant with the bitplanes in registers
 };
 
 // One image of a batch, as the device sees it.
@@ -106,9 +107,11 @@ hipError_t noalloc_scratch(void** ptr);
 int planes_for(int nbp);
 // Bitplane scratch of one wave: 2 row halves x MAXP planes x 64 lanes x 8 bytes.
 constexpr size_t scratch_bytes_per_wave(int maxp) { return (size_t)2 * maxp * 64 * 8; }
-// Waves of the per-image kernel that fit on one CU (occupancy query).
-int codec_resident_waves_per_cu(int decode, int cpass3, int maxp);
-hipError_t launch_codec(int decode, int cpass3, int maxp, const KArgs& a, int grid_waves, hipStream_t stream);
+// Waves of the per-image kernel that fit on one CU (occupancy query); reg_planes
+// selects the variant that keeps the bitplanes in registers (no scratch).
+int codec_resident_waves_per_cu(int decode, int cpass3, int maxp, int reg_planes);
+hipError_t launch_codec(int decode, int cpass3, int maxp, int reg_planes, const KArgs& a, int grid_waves,
+                        hipStream_t stream);
 hipError_t launch_queue_build(const QueueArgs& a, hipStream_t stream);
 // vgpr_cap_waves: 0 (compiler's register budget) or 5..8 (registers capped for that many waves per SIMD)
 hipError_t launch_batch(int decode, int maxp, const BatchArgs& a, int grid_blocks, int waves_per_block,

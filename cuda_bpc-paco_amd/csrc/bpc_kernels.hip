@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 
 #include "bpc_internal.h"
 
@@ -668,13 +669,86 @@ struct Scratch {
     out[0] = (u64)lo.x | ((u64)hi.x << 32);
     out[1] = (u64)lo.y | ((u64)hi.y << 32);
   }
+  // rows of half h (0: rows 0-31, 1: rows 32-63) of plane p, both columns
+  __device__ __forceinline__ void put_half(int h, int p, u32 even, u32 odd) { store(h * MAXP + p, even, odd); }
+  __device__ __forceinline__ uint2 half_plane(int h, int p) const { return load(h * MAXP + p); }
+  // all rows of plane bp, both columns
+  __device__ __forceinline__ void put_plane(int bp, u64 even, u64 odd) {
+    store(bp, (u32)even, (u32)odd);
+    store(MAXP + bp, (u32)(even >> 32), (u32)(odd >> 32));
+  }
+};
+
+// Plane k of a register array, read by a wave-uniform index with a switch (static
+// register indices in every case).
+template <int MAXP>
+__device__ __forceinline__ void plane_get(const u64 (&pl)[MAXP][2], int k, u64& e, u64& o) {
+  static_assert(MAXP <= 20, "register planes hold at most 20 planes");
+  e = o = 0;
+  switch (k) {
+      case 0: if (0 < MAXP) { e = pl[0 < MAXP ? 0 : 0][0]; o = pl[0 < MAXP ? 0 : 0][1]; } break;
+      case 1: if (1 < MAXP) { e = pl[1 < MAXP ? 1 : 0][0]; o = pl[1 < MAXP ? 1 : 0][1]; } break;
+      case 2: if (2 < MAXP) { e = pl[2 < MAXP ? 2 : 0][0]; o = pl[2 < MAXP ? 2 : 0][1]; } break;
+      case 3: if (3 < MAXP) { e = pl[3 < MAXP ? 3 : 0][0]; o = pl[3 < MAXP ? 3 : 0][1]; } break;
+      case 4: if (4 < MAXP) { e = pl[4 < MAXP ? 4 : 0][0]; o = pl[4 < MAXP ? 4 : 0][1]; } break;
+      case 5: if (5 < MAXP) { e = pl[5 < MAXP ? 5 : 0][0]; o = pl[5 < MAXP ? 5 : 0][1]; } break;
+      case 6: if (6 < MAXP) { e = pl[6 < MAXP ? 6 : 0][0]; o = pl[6 < MAXP ? 6 : 0][1]; } break;
+      case 7: if (7 < MAXP) { e = pl[7 < MAXP ? 7 : 0][0]; o = pl[7 < MAXP ? 7 : 0][1]; } break;
+      case 8: if (8 < MAXP) { e = pl[8 < MAXP ? 8 : 0][0]; o = pl[8 < MAXP ? 8 : 0][1]; } break;
+      case 9: if (9 < MAXP) { e = pl[9 < MAXP ? 9 : 0][0]; o = pl[9 < MAXP ? 9 : 0][1]; } break;
+      case 10: if (10 < MAXP) { e = pl[10 < MAXP ? 10 : 0][0]; o = pl[10 < MAXP ? 10 : 0][1]; } break;
+      case 11: if (11 < MAXP) { e = pl[11 < MAXP ? 11 : 0][0]; o = pl[11 < MAXP ? 11 : 0][1]; } break;
+      case 12: if (12 < MAXP) { e = pl[12 < MAXP ? 12 : 0][0]; o = pl[12 < MAXP ? 12 : 0][1]; } break;
+      case 13: if (13 < MAXP) { e = pl[13 < MAXP ? 13 : 0][0]; o = pl[13 < MAXP ? 13 : 0][1]; } break;
+      case 14: if (14 < MAXP) { e = pl[14 < MAXP ? 14 : 0][0]; o = pl[14 < MAXP ? 14 : 0][1]; } break;
+      case 15: if (15 < MAXP) { e = pl[15 < MAXP ? 15 : 0][0]; o = pl[15 < MAXP ? 15 : 0][1]; } break;
+      case 16: if (16 < MAXP) { e = pl[16 < MAXP ? 16 : 0][0]; o = pl[16 < MAXP ? 16 : 0][1]; } break;
+      case 17: if (17 < MAXP) { e = pl[17 < MAXP ? 17 : 0][0]; o = pl[17 < MAXP ? 17 : 0][1]; } break;
+      case 18: if (18 < MAXP) { e = pl[18 < MAXP ? 18 : 0][0]; o = pl[18 < MAXP ? 18 : 0][1]; } break;
+      case 19: if (19 < MAXP) { e = pl[19 < MAXP ? 19 : 0][0]; o = pl[19 < MAXP ? 19 : 0][1]; } break;
+      default: break;
+  }
+}
+// Writes plane k by a compare on every index (a switch here would move the array
+// to the stack).
+template <int MAXP>
+__device__ __forceinline__ void plane_put(u64 (&pl)[MAXP][2], int k, u64 e, u64 o) {
+#pragma unroll
+  for (int j = 0; j < MAXP; ++j)
+    if (j == k) {
+      pl[j][0] = e;
+      pl[j][1] = o;
+    }
+}
+
+// The same planes in registers: one 64-bit row mask per plane and column (64
+// VGPRs at 16 planes), for the per-image launch when every wave it needs is
+// resident at once (it then runs at two waves per SIMD at most, with registers to
+// spare).  Planes are read and written by a wave-uniform index (plane_get /
+// plane_put), so that the array keeps static register indices.
+template <int MAXP>
+struct RegPlanes {
+  u64 pl[MAXP][2];
+  __device__ __forceinline__ RegPlanes(void*, int) {
+#pragma unroll
+    for (int p = 0; p < MAXP; ++p) pl[p][0] = pl[p][1] = 0;
+  }
+  __device__ __forceinline__ void put_half(int h, int p, u32 even, u32 odd) {
+    pl[p][0] |= (u64)even << (32 * h);
+    pl[p][1] |= (u64)odd << (32 * h);
+  }
+  __device__ __forceinline__ uint2 half_plane(int h, int p) const {
+    return make_uint2((u32)(pl[p][0] >> (32 * h)), (u32)(pl[p][1] >> (32 * h)));
+  }
+  __device__ __forceinline__ void plane(int bp, u64* out) const { plane_get<MAXP>(pl, bp, out[0], out[1]); }
+  __device__ __forceinline__ void put_plane(int bp, u64 even, u64 odd) { plane_put<MAXP>(pl, bp, even, odd); }
 };
 
 // Coefficients of one lane's stripe (int32 or int16, two's complement) into the
 // wave's bitplane scratch and the sign masks; returns the OR of the magnitudes.
-template <typename T, int MAXP>
-__device__ __forceinline__ u32 read_planes(cgi32 src32, int64_t cbase, int rs, int rows, bool valid,
-                                           const Scratch<MAXP>& scr, LaneState<MAXP>& s) {
+template <typename T, int MAXP, typename PS>
+__device__ __forceinline__ u32 read_planes(cgi32 src32, int64_t cbase, int rs, int rows, bool valid, PS& ps,
+                                           LaneState<MAXP>& s) {
   const gptr<const T> src = reinterpret_cast<gptr<const T>>(src32);
   u32 mor = 0;
 #pragma unroll
@@ -713,9 +787,9 @@ __device__ __forceinline__ u32 read_planes(cgi32 src32, int64_t cbase, int rs, i
     }
     s.G[0] |= (u64)glo0 << (32 * h);
     s.G[1] |= (u64)glo1 << (32 * h);
-    // plane p, rows of this half, both columns -> the wave's scratch
+    // plane p, rows of this half, both columns -> the plane store
 #pragma unroll
-    for (int p = 0; p < MAXP; ++p) scr.store(h * MAXP + p, plo[p][0], plo[p][1]);
+    for (int p = 0; p < MAXP; ++p) ps.put_half(h, p, plo[p][0], plo[p][1]);
   }
   return mor;
 }
@@ -724,11 +798,12 @@ __device__ __forceinline__ u32 read_planes(cgi32 src32, int64_t cbase, int rs, i
 // TRAIN = 1 (with DEC = 0): symbol-count mode.  The encoder's modelling runs
 // unchanged, but every symbol is counted at the LUT entry whose probability would
 // code it (hist, in LDS) instead of being coded; nothing is written to dst.
-template <int DEC, int CP3, int MAXP, int TRAIN = 0, int BF = 0>
+// RP = 1: the bitplanes in registers (RegPlanes) instead of the wave's scratch slot.
+template <int DEC, int CP3, int MAXP, int TRAIN = 0, int BF = 0, int RP = 0>
 __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uint32_t* status_word, void* scr_base,
                                           unsigned long long* hist = nullptr) {
   const int lane = threadIdx.x & 63;
-  const Scratch<MAXP> scr(scr_base, lane);
+  typename std::conditional<RP != 0, RegPlanes<MAXP>, Scratch<MAXP>>::type scr(scr_base, lane);
   const int half = lane >> 5;
   const int hl = lane & 31;
   const int cb = a.cb;
@@ -843,7 +918,7 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
   u64 plane[2] = {0, 0};
   // (the per-image kernel prefetches each next plane; the batched kernel, at 8 waves
   // per SIMD, loads each plane when it starts and spends no registers on a prefetch)
-  if (!DEC && BF && top >= 0) scr.plane(top, plane);
+  if (!DEC && BF && !RP && top >= 0) scr.plane(top, plane);
 
   // Bitplane driver (Encode / Decode, BPC_PaCo.cu:968-1090).  In 3-pass mode the
   // reference's opening cleanup pass at MSB equals plane MSB with SPP and MRP
@@ -867,8 +942,8 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
     }
     if (lbad) status |= ST_RANGE;
     u64 next[2] = {0, 0};
-    if (!DEC && BF && bp > 0) scr.plane(bp - 1, next);
-    if (!DEC && !BF) scr.plane(bp, plane);
+    if (!DEC && BF && !RP && bp > 0) scr.plane(bp - 1, next);
+    if (!DEC && (!BF || RP)) scr.plane(bp, plane);
     const bool act_sr = act && !(CP3 && bp == msb);
     if (ballot(act_sr)) {
       sig_pass<DEC, CP3, 0, TRAIN, BF>(s, act_sr, plane, pl, g, a.src, a.dst);
@@ -876,13 +951,12 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
     }
     if (CP3) sig_pass<DEC, CP3, 1, TRAIN, BF>(s, act, plane, pl, g, a.src, a.dst);
     if (!DEC) {
-      if (BF) {
+      if (BF && !RP) {
         plane[0] = next[0];
         plane[1] = next[1];
       }
     } else {  // decoded magnitude bits of plane bp -> the wave's scratch
-      scr.store(bp, (u32)s.cur[0], (u32)s.cur[1]);
-      scr.store(MAXP + bp, (u32)(s.cur[0] >> 32), (u32)(s.cur[1] >> 32));
+      scr.put_plane(bp, s.cur[0], s.cur[1]);
       s.cur[0] = s.cur[1] = 0;
     }
   }
@@ -908,7 +982,7 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
         if (r0 >= r1) break;
         uint2 pw[MAXP];  // planes 0..top of this half's rows; unwritten planes read as 0
 #pragma unroll
-        for (int p = 0; p < MAXP; ++p) pw[p] = p <= top ? scr.load(h * MAXP + p) : make_uint2(0u, 0u);
+        for (int p = 0; p < MAXP; ++p) pw[p] = p <= top ? scr.half_plane(h, p) : make_uint2(0u, 0u);
         const u32 g0 = (u32)(s.G[0] >> (32 * h)), g1 = (u32)(s.G[1] >> (32 * h));
         for (int r = r0; r < r1; ++r) {
           const int sh = r - r0;
@@ -939,13 +1013,14 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
 }
 
 
-template <int DEC, int CP3, int MAXP>
+// RP = 1: bitplanes in registers (no scratch slot; a.scratch unused).
+template <int DEC, int CP3, int MAXP, int RP>
 __global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
   __shared__ uint8_t slut[256];
   const lds_u8 sl = build_sign_lut(slut);
   const int wave = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   const int nwaves = (int)((gridDim.x * blockDim.x) >> 6);
-  void* scr = reinterpret_cast<char*>(a.scratch) + (size_t)wave * (2 * MAXP * 64 * 8);
+  void* scr = RP ? nullptr : reinterpret_cast<char*>(a.scratch) + (size_t)wave * (2 * MAXP * 64 * 8);
   HalfDesc d;
   d.src = (cgi32)a.src;
   d.dst = (gi32)a.dst;
@@ -969,7 +1044,7 @@ __global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
     const int cb = a.single ? (half ? a.ncb : unit) : 2 * unit + half;
     d.cb = cb < a.ncb ? cb : -1;
     // every wave of this launch may be the critical path: branch-free coder steps
-    code_pair<DEC, CP3, MAXP, 0, 1>(d, a.rows, a.status, scr);
+    code_pair<DEC, CP3, MAXP, 0, 1, RP>(d, a.rows, a.status, scr);
   }
 }
 
@@ -1181,43 +1256,49 @@ int planes_for(int nbp) {
   return 28;
 }
 
-template <int DEC, int CP3, int MAXP>
+template <int DEC, int CP3, int MAXP, int RP>
 static hipError_t launch_one(const KArgs& a, int grid_waves, hipStream_t stream) {
-  hipLaunchKernelGGL((bpc_codec_kernel<DEC, CP3, MAXP>), dim3(grid_waves), dim3(64), 0, stream, a);
+  hipLaunchKernelGGL((bpc_codec_kernel<DEC, CP3, MAXP, RP>), dim3(grid_waves), dim3(64), 0, stream, a);
   return hipGetLastError();
 }
 
-template <int DEC, int CP3, int MAXP>
+template <int DEC, int CP3, int MAXP, int RP>
 static int plain_resident() {
   int blocks = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, bpc_codec_kernel<DEC, CP3, MAXP>, 64, 0) != hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, bpc_codec_kernel<DEC, CP3, MAXP, RP>, 64, 0) !=
+      hipSuccess)
     return 0;
   return blocks;
 }
 
-template <int DEC, int CP3>
+// Register planes exist for 16 and 20 planes (28 would not stay in registers).
+template <int DEC, int CP3, int RP>
 static hipError_t launch_p(int maxp, const KArgs& a, int grid_waves, hipStream_t stream) {
   switch (maxp) {
-    case 16: return launch_one<DEC, CP3, 16>(a, grid_waves, stream);
-    case 20: return launch_one<DEC, CP3, 20>(a, grid_waves, stream);
-    case 28: return launch_one<DEC, CP3, 28>(a, grid_waves, stream);
+    case 16: return launch_one<DEC, CP3, 16, RP>(a, grid_waves, stream);
+    case 20: return launch_one<DEC, CP3, 20, RP>(a, grid_waves, stream);
+    case 28: return RP ? hipErrorInvalidValue : launch_one<DEC, CP3, 28, 0>(a, grid_waves, stream);
     default: return hipErrorInvalidValue;
   }
 }
 
-template <int DEC, int CP3>
+template <int DEC, int CP3, int RP>
 static int resident_p(int maxp) {
   switch (maxp) {
-    case 16: return plain_resident<DEC, CP3, 16>();
-    case 20: return plain_resident<DEC, CP3, 20>();
-    case 28: return plain_resident<DEC, CP3, 28>();
+    case 16: return plain_resident<DEC, CP3, 16, RP>();
+    case 20: return plain_resident<DEC, CP3, 20, RP>();
+    case 28: return RP ? 0 : plain_resident<DEC, CP3, 28, 0>();
     default: return 0;
   }
 }
 
-int codec_resident_waves_per_cu(int decode, int cpass3, int maxp) {
-  if (decode) return cpass3 ? resident_p<1, 1>(maxp) : resident_p<1, 0>(maxp);
-  return cpass3 ? resident_p<0, 1>(maxp) : resident_p<0, 0>(maxp);
+int codec_resident_waves_per_cu(int decode, int cpass3, int maxp, int reg_planes) {
+  if (reg_planes) {
+    if (decode) return cpass3 ? resident_p<1, 1, 1>(maxp) : resident_p<1, 0, 1>(maxp);
+    return cpass3 ? resident_p<0, 1, 1>(maxp) : resident_p<0, 0, 1>(maxp);
+  }
+  if (decode) return cpass3 ? resident_p<1, 1, 0>(maxp) : resident_p<1, 0, 0>(maxp);
+  return cpass3 ? resident_p<0, 1, 0>(maxp) : resident_p<0, 0, 0>(maxp);
 }
 
 template <int DEC, int MAXP, int OCC>
@@ -1300,10 +1381,17 @@ hipError_t launch_batch(int decode, int maxp, const BatchArgs& a, int grid_block
   }
 }
 
-hipError_t launch_codec(int decode, int cpass3, int maxp, const KArgs& a, int grid_waves, hipStream_t stream) {
+hipError_t launch_codec(int decode, int cpass3, int maxp, int reg_planes, const KArgs& a, int grid_waves,
+                        hipStream_t stream) {
   if (a.ncb <= 0) return hipSuccess;
-  if (decode) return cpass3 ? launch_p<1, 1>(maxp, a, grid_waves, stream) : launch_p<1, 0>(maxp, a, grid_waves, stream);
-  return cpass3 ? launch_p<0, 1>(maxp, a, grid_waves, stream) : launch_p<0, 0>(maxp, a, grid_waves, stream);
+  if (reg_planes) {
+    if (decode)
+      return cpass3 ? launch_p<1, 1, 1>(maxp, a, grid_waves, stream) : launch_p<1, 0, 1>(maxp, a, grid_waves, stream);
+    return cpass3 ? launch_p<0, 1, 1>(maxp, a, grid_waves, stream) : launch_p<0, 0, 1>(maxp, a, grid_waves, stream);
+  }
+  if (decode)
+    return cpass3 ? launch_p<1, 1, 0>(maxp, a, grid_waves, stream) : launch_p<1, 0, 0>(maxp, a, grid_waves, stream);
+  return cpass3 ? launch_p<0, 1, 0>(maxp, a, grid_waves, stream) : launch_p<0, 0, 0>(maxp, a, grid_waves, stream);
 }
 
 }  // namespace bpc

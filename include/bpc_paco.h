@@ -134,11 +134,13 @@ int bpc_decode_device_i16(const int32_t* d_stream, int16_t* d_coeffs, int size_x
  * (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  bpc_encode_device (direction BPC_CODE) or bpc_decode_device
  * (BPC_DECODE, where fill_unused and d_ncw are ignored) with the same results, but
- * no call to hipMalloc, hipFree or any allocating routine: the bitplane scratch is a
- * fixed array of BPC_NOALLOC_SCRATCH_BYTES per device (held until the process
- * ends), and the launch uses at most as many waves as it has slots for (16 KiB per
- * wave for up to 16 LUT bitplanes, 20 KiB up to 20, 28 KiB beyond; any image size
- * is accepted, larger ones take more codeblocks per wave).  d_status is required.
+ * no call to hipMalloc, hipFree or any allocating routine.  Images whose waves all
+ * fit on the device at once keep their bitplanes in registers and use no scratch;
+ * larger ones use a fixed array of BPC_NOALLOC_SCRATCH_BYTES per device (held until
+ * the process ends) for the bitplane slots, and the launch uses at most as many
+ * waves as it has slots for (16 KiB per wave for up to 16 LUT bitplanes, 20 KiB up
+ * to 20, 28 KiB beyond; any image size is accepted, larger ones take more
+ * codeblocks per wave).  d_status is required.
  * Every call on one GPU shares that GPU's array: calls that overlap on one GPU (on
  * different streams, or from different threads) race and must be ordered by the
  * caller.  No part of the array needs clearing: each launch writes every slot word

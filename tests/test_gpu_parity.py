@@ -93,6 +93,16 @@ def test_plain_4096_3pass(bpc, O):
     check_codec(bpc, O, img, lut, 1, nbp=16, layout=1)
 
 
+@pytest.mark.parametrize("cp3,nbp,mag", [(1, 16, 16), (0, 20, 19)])
+def test_plain_6144_more_waves_than_resident(bpc, O, cp3, nbp, mag):
+    """6144^2 needs more waves than fit at once with the bitplanes in registers, so
+    the per-image launch keeps them in its scratch slots and grid-strides (16- and
+    20-plane kernels); 4096^2 above runs the register form."""
+    img, lut = trained(bpc, 6144, cp3, nbp, seed=250 + nbp, mag_bits=mag, ll_bits=min(13, mag),
+                       detail_exp=9 if mag <= 16 else mag - 6)
+    check_codec(bpc, O, img, lut, cp3, nbp=nbp, layout=1, stride=0 if mag <= 16 else 8192)
+
+
 @pytest.mark.parametrize("size_y", [2048, 2112])
 def test_one_codeblock_per_wave_and_pairs(bpc, O, size_y):
     """The per-image launch codes one codeblock per wave when the image has no more
