@@ -242,16 +242,16 @@ def bench_batched(args, check=True):
     # autotuner, per direction: codeblocks per workgroup (2 per wave) x resident
     # waves per SIMD x register budget (compiler's, or capped for 5..8 waves per SIMD)
     tune = []
-    for wpb in (1, 2, 4):
-        for wps in (0, 2, 3):
-            for cap in (0, 5, 6, 7, 8):
-                t = {"codeblocks_per_workgroup": 2 * wpb, "waves_per_simd": wps or "max",
-                     "vgpr_cap_waves_per_simd": cap}
-                use(t)
-                batched_step(graph=False)
-                e, d = batched_step(graph=False)
-                t.update({"encode_ms": round(e, 3), "decode_ms": round(d, 3)})
-                tune.append(t)
+    grid = [(wpb, wps, cap) for wpb in (1, 2, 4) for wps in (0, 2, 3) for cap in (0, 5, 6, 7, 8)]
+    if args.no_autotune:  # the setting the sweep picks on MI355X, for short (profiled) runs
+        grid = [(4, 0, 8)]
+    for wpb, wps, cap in grid:
+        t = {"codeblocks_per_workgroup": 2 * wpb, "waves_per_simd": wps or "max", "vgpr_cap_waves_per_simd": cap}
+        use(t)
+        batched_step(graph=False)
+        e, d = batched_step(graph=False)
+        t.update({"encode_ms": round(e, 3), "decode_ms": round(d, 3)})
+        tune.append(t)
     keys = ("codeblocks_per_workgroup", "waves_per_simd", "vgpr_cap_waves_per_simd")
     chosen["encode"] = {k: min(tune, key=lambda t: t["encode_ms"])[k] for k in keys}
     chosen["decode"] = {k: min(tune, key=lambda t: t["decode_ms"])[k] for k in keys}
@@ -798,6 +798,9 @@ def main():
     ap.add_argument("--detail-out", default=os.path.join(ROOT, "gpurun_out", "bench_detail.json"),
                     help="JSON file for the autotuner and served-rate tables ('' = none)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched workload: one setting (8 codeblocks per workgroup, all waves, register cap 8) "
+                         "instead of the 45-setting sweep")
     ap.add_argument("--traffic-child", default="", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.traffic_child:
