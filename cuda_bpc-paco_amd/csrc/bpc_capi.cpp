@@ -510,14 +510,15 @@ struct bpc_batch {
   long long max_cbs = 0;
   size_t stage_bytes = 0;          // run words + offsets (2*max_images+1) + descriptors (max_images)
   char* d_stage = nullptr;         // device copy of the staging area
-  char* h_stage = nullptr;         // pinned staging area
+  char* h_stage = nullptr;         // pinned staging area (host address)
+  const char* h_stage_dev = nullptr;  // its device address (read by the stage kernel)
   uint32_t* d_keys = nullptr;
   uint32_t* d_sorted = nullptr;
   uint32_t* d_pair_keys = nullptr;
   uint32_t* d_pair_queue = nullptr;
   uint32_t* d_wide = nullptr;      // pairs deferred to the 28-plane pass
   uint32_t* d_ctl = nullptr;
-  hipEvent_t staged = nullptr;  // the last staging upload has been consumed
+  hipEvent_t staged = nullptr;  // the stage kernel of the last run has read the pinned area
   bool staged_pending = false;
   hipEvent_t done = nullptr;  // the last run has finished
   bool done_pending = false;
@@ -527,6 +528,7 @@ struct bpc_batch {
   int waves_per_simd = 0;  // 0: as many as fit
   int vgpr_cap_waves = 0;  // 0: compiler's register budget; 5..8: capped for that many waves per SIMD
   int cus = 0;
+  int last_direction = 0, last_maxp = 0;  // of the last run (the deferred 28-plane pass codes its pairs)
   // captured launch sequences, least recently used first; at most kMaxGraphs
   static constexpr size_t kMaxGraphs = 32;
   std::list<std::pair<std::string, hipGraphExec_t>> graphs;
@@ -570,15 +572,13 @@ struct BatchPlan {
   long long total_cbs = 0, total_pairs = 0;
   int maxp = 16;
   int rows = 0;
-  std::string signature;  // every value the captured launch sequence depends on
 };
 
+// Fills the pinned staging area: run words, offset tables, image descriptors.
 int batch_stage(bpc_batch* b, int direction, const bpc_image* images, int n, BatchPlan& plan) {
   if (n <= 0 || n > b->max_images) return fail(BPC_ERR_ARG, "batch of %d images (capacity %d)", n, b->max_images);
   plan.rows = images[0].params.cblock_length;
   int max_nbp = 1;
-  char sig[160];
-  plan.signature = "d" + std::to_string(direction) + ";n" + std::to_string(n) + ";";
   long long pos = 0;
   bpc::BatchImage* hi = b->h_images();
   long long* ho = b->h_offsets();
@@ -608,22 +608,21 @@ int batch_stage(bpc_batch* b, int direction, const bpc_image* images, int n, Bat
     d.cpass3 = im.params.cpass3;
     d.pad = 0;
     ho[k] = plan.total_cbs;
-    ho[n + 1 + k] = pos;
+    ho[b->max_images + 1 + k] = pos;  // sorted_base: a fixed place for the batch object
     plan.total_cbs += ncb;
     pos += (ncb + 1) & ~1ll;
     max_nbp = std::max(max_nbp, im.params.lut_n_bitplanes);
-    // geometry, pass mode and bitplanes of every image: the queue build and the
-    // grid depend on them (pointers and LUTs are read from the uploaded descriptors)
-    snprintf(sig, sizeof sig, "%dx%d,c%d,l%d,b%d,o%d,s%d,f%d;", im.size_x, im.size_y, im.params.cpass3,
-             im.params.dwt_levels, im.params.lut_n_bitplanes, im.params.layout, d.stride, im.fill_unused);
-    plan.signature += sig;
   }
   ho[n] = plan.total_cbs;
   plan.total_pairs = pos / 2;
   if (plan.total_cbs > b->max_cbs) return fail(BPC_ERR_ARG, "batch of %lld codeblocks (capacity %lld)", plan.total_cbs, b->max_cbs);
   plan.maxp = bpc::planes_for(max_nbp);
-  snprintf(sig, sizeof sig, "r%d,cbs%lld,pairs%lld,p%d", plan.rows, plan.total_cbs, plan.total_pairs, plan.maxp);
-  plan.signature += sig;
+  bpc::BatchRun* r = b->h_run();
+  r->total_cbs = plan.total_cbs;
+  r->total_pairs = plan.total_pairs;
+  r->n_images = n;
+  r->rows = plan.rows;
+  r->stage_bytes = (uint32_t)(b->images_offset() + sizeof(bpc::BatchImage) * (size_t)n);
   return BPC_OK;
 }
 
@@ -636,49 +635,39 @@ int batch_resident_blocks(bpc_batch* b, int direction, int maxp) {
   return per_cu * b->cus;
 }
 
-// Workgroups of the persistent kernel: resident capacity, at most one wave per pair.
-int batch_grid_blocks(bpc_batch* b, int direction, int maxp, const BatchPlan& plan) {
-  if (plan.total_pairs <= 0) return 0;
+// Workgroups of the persistent kernel: the resident capacity; an eager run takes at
+// most one wave per pair (a captured sequence keeps the full grid, whose extra
+// waves find the queue drained and leave).
+int batch_grid_blocks(bpc_batch* b, int direction, int maxp, const BatchPlan& plan, bool captured) {
+  const int full = batch_resident_blocks(b, direction, maxp);
+  if (captured) return full;
   const int wpb = b->waves_per_block;
-  return (int)std::min<long long>(batch_resident_blocks(b, direction, maxp), (plan.total_pairs + wpb - 1) / wpb);
+  return (int)std::min<long long>(full, std::max<long long>(1, (plan.total_pairs + wpb - 1) / wpb));
 }
 
 // Workgroups of the 28-plane pass over the deferred pairs (rare: a codeblock whose
-// MSB is at least the main pass's planes): one per CU.
-int batch_wide_blocks(bpc_batch* b, int maxp, const BatchPlan& plan) {
-  if (maxp >= 28 || plan.total_pairs <= 0) return 0;
+// MSB is at least the main pass's planes): one per CU.  Its workgroups leave at
+// once when nothing was deferred.
+int batch_wide_blocks(bpc_batch* b, int maxp, const BatchPlan& plan, bool captured) {
+  if (maxp >= 28) return 0;
+  if (captured) return b->cus;
   return (int)std::max<long long>(1, std::min<long long>(b->cus, (plan.total_pairs + b->waves_per_block - 1) /
                                                                      b->waves_per_block));
 }
 
-int batch_upload(bpc_batch* b, int n, uint32_t* d_status, hipStream_t stream) {
+// The run's first kernel reads the pinned staging area into device memory; once it
+// has, the host may fill the area for the next run.
+int batch_upload(bpc_batch* b, uint32_t* d_status, uint32_t* host_status, hipStream_t stream) {
   b->h_run()->status = d_status;
-  const size_t bytes = b->images_offset() + sizeof(bpc::BatchImage) * (size_t)n;
-  HIP_TRY(hipMemcpyAsync(b->d_stage, b->h_stage, bytes, hipMemcpyHostToDevice, stream));
+  b->h_run()->host_status = host_status;
+  hipError_t e = bpc::launch_batch_stage(b->h_stage_dev, b->d_stage, b->d_ctl, stream);
+  if (e != hipSuccess) return hip_fail(e, "batch stage kernel");
   HIP_TRY(hipEventRecord(b->staged, stream));
   b->staged_pending = true;
   return BPC_OK;
 }
 
-// Kernel sequence of one batch run (captured into a graph when asked).
-int batch_kernels(bpc_batch* b, int direction, int n, const BatchPlan& plan, int maxp, int blocks, int wide_blocks,
-                  hipStream_t stream) {
-  bpc::QueueArgs q;
-  q.images = b->d_images();
-  q.cb_offset = b->d_offsets();
-  q.sorted_base = b->d_offsets() + n + 1;
-  q.n_images = n;
-  q.total_cbs = plan.total_cbs;
-  q.total_pairs = plan.total_pairs;
-  q.rows = plan.rows;
-  q.decode = direction == BPC_DECODE;
-  q.keys = b->d_keys;
-  q.sorted = b->d_sorted;
-  q.pair_keys = b->d_pair_keys;
-  q.pair_queue = b->d_pair_queue;
-  q.ctl = b->d_ctl;
-  hipError_t e = bpc::launch_queue_build(q, stream);
-  if (e != hipSuccess) return hip_fail(e, "queue build");
+bpc::BatchArgs batch_args(bpc_batch* b) {
   bpc::BatchArgs a;
   a.scratch = b->d_scratch;
   a.images = b->d_images();
@@ -687,24 +676,162 @@ int batch_kernels(bpc_batch* b, int direction, int n, const BatchPlan& plan, int
   a.pair_keys = b->d_pair_keys;
   a.wide = b->d_wide;
   a.ctl = b->d_ctl;
-  a.n_pairs = plan.total_pairs;
-  a.rows = plan.rows;
+  a.run = b->d_run();
   a.wide_pass = 0;
-  if (blocks > 0) {
-    e = bpc::launch_batch(direction == BPC_DECODE, maxp, a, blocks, b->waves_per_block, b->vgpr_cap_waves, stream);
-    if (e != hipSuccess) return hip_fail(e, "batched kernel launch");
-  }
-  if (wide_blocks > 0) {
+  a.fold = 1;
+  a.report_wide = 0;
+  return a;
+}
+
+// Kernel sequence of one run after the stage kernel (captured into a graph when
+// asked): queue build, main pass, 28-plane pass (unless deferred); the last wave
+// of the last launch folds the run's status.
+// Nothing in it depends on the run's shape but maxp: sizes are read from the run
+// words, grids from the device.
+int batch_kernels(bpc_batch* b, int direction, int maxp, int blocks, int wide_blocks, bool defer_wide,
+                  hipStream_t stream) {
+  bpc::QueueArgs q;
+  q.run = b->d_run();
+  q.images = b->d_images();
+  q.cb_offset = b->d_offsets();
+  q.sorted_base = b->d_offsets() + b->max_images + 1;
+  q.decode = direction == BPC_DECODE;
+  q.max_images = b->max_images;
+  q.cus = b->cus;
+  q.keys = b->d_keys;
+  q.sorted = b->d_sorted;
+  q.pair_keys = b->d_pair_keys;
+  q.pair_queue = b->d_pair_queue;
+  q.ctl = b->d_ctl;
+  hipError_t e = bpc::launch_queue_build(q, stream);
+  if (e != hipSuccess) return hip_fail(e, "queue build");
+  bpc::BatchArgs a = batch_args(b);
+  const bool wide = wide_blocks > 0 && !defer_wide;
+  a.fold = wide ? 0 : 1;  // the last launch of the run folds its status
+  a.report_wide = defer_wide && wide_blocks > 0;
+  e = bpc::launch_batch(direction == BPC_DECODE, maxp, a, blocks, b->waves_per_block, b->vgpr_cap_waves, stream);
+  if (e != hipSuccess) return hip_fail(e, "batched kernel launch");
+  if (wide) {
     a.wide_pass = 1;
+    a.fold = 1;
+    a.report_wide = 0;
     e = bpc::launch_batch(direction == BPC_DECODE, 28, a, wide_blocks, b->waves_per_block, 0, stream);
     if (e != hipSuccess) return hip_fail(e, "batched kernel launch (28 planes)");
   }
-  e = bpc::launch_status_fold(b->d_ctl, b->d_run(), stream);
-  if (e != hipSuccess) return hip_fail(e, "status fold");
+  return BPC_OK;
+}
+
+// The captured sequence for (direction, maxp, tuning, deferral), captured on first use.
+int batch_graph(bpc_batch* b, int direction, int maxp, int blocks, int wide_blocks, bool defer_wide,
+                hipGraphExec_t* out) {
+  char key[96];
+  snprintf(key, sizeof key, "d%d,p%d,wpb%d,wps%d,cap%d,g%d,w%d,f%d", direction, maxp, b->waves_per_block,
+           b->waves_per_simd, b->vgpr_cap_waves, blocks, wide_blocks, defer_wide ? 1 : 0);
+  auto it = std::find_if(b->graphs.begin(), b->graphs.end(), [&](const auto& kv) { return kv.first == key; });
+  if (it != b->graphs.end()) {
+    b->graphs.splice(b->graphs.end(), b->graphs, it);  // most recently used last
+    *out = b->graphs.back().second;
+    return BPC_OK;
+  }
+  hipStream_t cap = nullptr;
+  HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+  hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(cap);
+    return hip_fail(e, "graph capture");
+  }
+  int st = batch_kernels(b, direction, maxp, blocks, wide_blocks, defer_wide, cap);
+  hipGraph_t graph = nullptr;
+  e = hipStreamEndCapture(cap, &graph);
+  (void)hipStreamDestroy(cap);
+  if (st) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return st;
+  }
+  if (e != hipSuccess) return hip_fail(e, "graph capture");
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return hip_fail(e, "graph instantiate");
+  if (b->graphs.size() >= bpc_batch::kMaxGraphs) {  // evict the least recently used, once idle
+    if (b->done_pending) HIP_TRY(hipEventSynchronize(b->done));
+    (void)hipGraphExecDestroy(b->graphs.front().second);
+    b->graphs.pop_front();
+  }
+  b->graphs.emplace_back(key, exec);
+  *out = exec;
   return BPC_OK;
 }
 
 }  // namespace
+
+namespace bpc {
+
+int batch_run_ex(bpc_batch* b, int direction, const bpc_image* images, int n_images, int use_graph,
+                 uint32_t* d_status, uint32_t* host_status, int defer_wide, void* stream) {
+  if (!b || !images) return fail(BPC_ERR_ARG, "null batch or images");
+  if (direction != BPC_CODE && direction != BPC_DECODE) return fail(BPC_ERR_ARG, "bad direction");
+  std::lock_guard<std::mutex> lk(b->mu);
+  // the pinned staging area may still be read by the previous run's stage kernel
+  if (b->staged_pending) HIP_TRY(hipEventSynchronize(b->staged));
+  BatchPlan plan;
+  int st = batch_stage(b, direction, images, n_images, plan);
+  if (st) return st;
+  hipStream_t s = (hipStream_t)stream;
+  const int maxp = plan.maxp;
+  const bool captured = use_graph != 0;
+  const int blocks = batch_grid_blocks(b, direction, maxp, plan, captured);
+  const int wide_blocks = batch_wide_blocks(b, maxp, plan, captured);
+  // Bitplane slots for the full resident grid of this tuning (and the 28-plane
+  // pass): grown once, when the previous run is done, so that larger batches later
+  // do not grow it again (a growth drops the captured graphs, which hold the old
+  // address).
+  const size_t full = std::max((size_t)batch_resident_blocks(b, direction, maxp) * b->waves_per_block *
+                                   bpc::scratch_bytes_per_wave(maxp),
+                               (size_t)b->cus * b->waves_per_block * bpc::scratch_bytes_per_wave(28));
+  if (full > b->scratch_bytes) {
+    if (b->done_pending) HIP_TRY(hipEventSynchronize(b->done));
+    drop_graphs(b);
+    if (b->d_scratch) HIP_TRY(hipFree(b->d_scratch));
+    b->d_scratch = nullptr;
+    b->scratch_bytes = 0;
+    HIP_TRY(hipMalloc(&b->d_scratch, full));
+    b->scratch_bytes = full;
+  }
+  hipGraphExec_t exec = nullptr;
+  if (captured) {  // before the upload: a failed capture leaves no half-run on the stream
+    st = batch_graph(b, direction, maxp, blocks, wide_blocks, defer_wide != 0, &exec);
+    if (st) return st;
+  }
+  st = batch_upload(b, d_status, host_status, s);
+  if (st) return st;
+  if (captured)
+    HIP_TRY(hipGraphLaunch(exec, s));
+  else
+    st = batch_kernels(b, direction, maxp, blocks, wide_blocks, defer_wide != 0, s);
+  if (st) return st;
+  b->last_direction = direction;
+  b->last_maxp = maxp;
+  HIP_TRY(hipEventRecord(b->done, s));
+  b->done_pending = true;
+  return BPC_OK;
+}
+
+int batch_run_wide(bpc_batch* b, void* stream) {
+  if (!b) return fail(BPC_ERR_ARG, "null batch");
+  std::lock_guard<std::mutex> lk(b->mu);
+  if (b->last_maxp >= 28 || b->last_maxp == 0) return BPC_OK;
+  hipStream_t s = (hipStream_t)stream;
+  bpc::BatchArgs a = batch_args(b);
+  a.wide_pass = 1;
+  hipError_t e = bpc::launch_batch(b->last_direction == BPC_DECODE, 28, a, b->cus, b->waves_per_block, 0, s);
+  if (e != hipSuccess) return hip_fail(e, "batched kernel launch (28 planes)");
+  HIP_TRY(hipEventRecord(b->done, s));
+  b->done_pending = true;
+  return BPC_OK;
+}
+
+}  // namespace bpc
 
 extern "C" {
 
@@ -723,6 +850,9 @@ int bpc_batch_create(bpc_batch** out, int max_images, long long max_codeblocks) 
   b->stage_bytes = b->images_offset() + sizeof(bpc::BatchImage) * (size_t)max_images;
   if (e == hipSuccess) e = hipMalloc(&b->d_stage, b->stage_bytes);
   if (e == hipSuccess) e = hipHostMalloc(&b->h_stage, b->stage_bytes, hipHostMallocDefault);
+  void* h_dev = nullptr;
+  if (e == hipSuccess) e = hipHostGetDevicePointer(&h_dev, b->h_stage, 0);
+  b->h_stage_dev = static_cast<const char*>(h_dev);
   if (e == hipSuccess) e = hipMalloc(&b->d_keys, sizeof(uint32_t) * max_codeblocks);
   if (e == hipSuccess) e = hipMalloc(&b->d_sorted, sizeof(uint32_t) * entries);
   if (e == hipSuccess) e = hipMalloc(&b->d_pair_keys, sizeof(uint32_t) * (entries / 2 + 1));
@@ -775,85 +905,11 @@ int bpc_batch_graph_count(bpc_batch* b) {
 
 int bpc_batch_run(bpc_batch* b, int direction, const bpc_image* images, int n_images, int use_graph,
                   uint32_t* d_status, void* stream) {
-  if (!b || !images) return fail(BPC_ERR_ARG, "null batch or images");
-  if (direction != BPC_CODE && direction != BPC_DECODE) return fail(BPC_ERR_ARG, "bad direction");
-  std::lock_guard<std::mutex> lk(b->mu);
   if (!d_status) {
     int st = default_status(&d_status);
     if (st) return st;
   }
-  // the pinned staging area may still be read by the previous run's upload
-  if (b->staged_pending) HIP_TRY(hipEventSynchronize(b->staged));
-  BatchPlan plan;
-  int st = batch_stage(b, direction, images, n_images, plan);
-  if (st) return st;
-  hipStream_t s = (hipStream_t)stream;
-  const int maxp = plan.maxp;
-  const int blocks = batch_grid_blocks(b, direction, maxp, plan);
-  const int wide_blocks = batch_wide_blocks(b, maxp, plan);
-  const size_t need = std::max((size_t)blocks * b->waves_per_block * bpc::scratch_bytes_per_wave(maxp),
-                               (size_t)wide_blocks * b->waves_per_block * bpc::scratch_bytes_per_wave(28));
-  if (need > b->scratch_bytes) {
-    // Grow once the previous run is done, to what the full resident grid of this
-    // tuning needs, so that larger batches later do not grow it again (a growth
-    // drops the captured graphs, which hold the old address).
-    const size_t full = std::max((size_t)batch_resident_blocks(b, direction, maxp) * b->waves_per_block *
-                                     bpc::scratch_bytes_per_wave(maxp),
-                                 (size_t)b->cus * b->waves_per_block * bpc::scratch_bytes_per_wave(28));
-    if (b->done_pending) HIP_TRY(hipEventSynchronize(b->done));
-    drop_graphs(b);
-    if (b->d_scratch) HIP_TRY(hipFree(b->d_scratch));
-    b->d_scratch = nullptr;
-    b->scratch_bytes = 0;
-    HIP_TRY(hipMalloc(&b->d_scratch, std::max(need, full)));
-    b->scratch_bytes = std::max(need, full);
-  }
-  st = batch_upload(b, n_images, d_status, s);
-  if (st) return st;
-  if (!use_graph) {
-    st = batch_kernels(b, direction, n_images, plan, maxp, blocks, wide_blocks, s);
-  } else {
-    char tun[96];
-    snprintf(tun, sizeof tun, "|wpb%d,wps%d,cap%d,g%d,w%d", b->waves_per_block, b->waves_per_simd, b->vgpr_cap_waves,
-             blocks, wide_blocks);
-    const std::string key = plan.signature + tun;
-    auto it = std::find_if(b->graphs.begin(), b->graphs.end(), [&](const auto& kv) { return kv.first == key; });
-    if (it != b->graphs.end()) {
-      b->graphs.splice(b->graphs.end(), b->graphs, it);  // most recently used last
-    } else {
-      hipStream_t cap = nullptr;
-      HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-      hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
-      if (e != hipSuccess) {
-        (void)hipStreamDestroy(cap);
-        return hip_fail(e, "graph capture");
-      }
-      st = batch_kernels(b, direction, n_images, plan, maxp, blocks, wide_blocks, cap);
-      hipGraph_t graph = nullptr;
-      e = hipStreamEndCapture(cap, &graph);
-      (void)hipStreamDestroy(cap);
-      if (st) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return st;
-      }
-      if (e != hipSuccess) return hip_fail(e, "graph capture");
-      hipGraphExec_t exec = nullptr;
-      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      if (e != hipSuccess) return hip_fail(e, "graph instantiate");
-      if (b->graphs.size() >= bpc_batch::kMaxGraphs) {  // evict the least recently used, once idle
-        if (b->done_pending) HIP_TRY(hipEventSynchronize(b->done));
-        (void)hipGraphExecDestroy(b->graphs.front().second);
-        b->graphs.pop_front();
-      }
-      b->graphs.emplace_back(key, exec);
-    }
-    HIP_TRY(hipGraphLaunch(b->graphs.back().second, s));
-  }
-  if (st) return st;
-  HIP_TRY(hipEventRecord(b->done, s));
-  b->done_pending = true;
-  return BPC_OK;
+  return bpc::batch_run_ex(b, direction, images, n_images, use_graph, d_status, nullptr, 0, stream);
 }
 
 }  // extern "C"

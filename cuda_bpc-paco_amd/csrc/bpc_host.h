@@ -18,6 +18,9 @@ enum : uint32_t {
   ST_MALFORMED = 4u,  // decoder: MSB word not in [0,27] and not 32
   ST_WIDE = 8u,       // a codeblock needs more bitplanes than the instantiated kernel holds
   ST_NARROW = 16u,    // decoder: a coefficient outside the int16 output range (saturated)
+  // batched codec, host-visible word only: pairs wait for the 28-plane pass that
+  // the run left out (the server launches it when it sees this bit)
+  ST_WIDE_PENDING = 0x80000000u,
 };
 
 // Sets the calling thread's message (bpc_last_error) and returns code.

@@ -3,6 +3,7 @@
 #include <stdint.h>
 #include <hip/hip_runtime_api.h>
 
+#include "../../include/bpc_paco.h"
 #include "bpc_host.h"
 
 namespace bpc {
@@ -46,27 +47,37 @@ enum : int {
   CTL_STATUS = 65,      // status bits of the batch
   CTL_WIDE_COUNT = 66,  // pairs deferred to the 28-plane pass (MSB beyond the main pass's planes)
   CTL_WIDE_HEAD = 67,   // next deferred pair to hand out
+  CTL_DONE = 68,        // waves of the main pass that have finished
+  CTL_DONE_WIDE = 69,   // waves of the 28-plane pass that have finished
   CTL_IMG = 128,        // per image: 29 MSB-bucket counts, then cursors (32 words each)
 };
 
 constexpr int NBUCKET = 29;  // MSB buckets: 27..0, then all-zero / rejected
 
-// Per-run words of a batch, uploaded with the image descriptors (so that a
-// captured launch sequence reads them at replay time).
+// Per-run words of a batch, at the head of the staging area: the kernels of a
+// run read the sizes from here rather than from their arguments, so that one
+// captured launch sequence serves batches of any shape.
 struct BatchRun {
-  uint32_t* status;  // the caller's status word
-  uint64_t pad[7];
+  uint32_t* status;       // the caller's status word (device): OR of the run's ST_* bits
+  uint32_t* host_status;  // nullable, host-visible: the run's ST_* bits, stored by the fold
+  int64_t total_cbs;      // codeblocks over all images
+  int64_t total_pairs;    // queue entries (codeblock pairs, each image's count rounded up to even)
+  int32_t n_images;
+  int32_t rows;           // codeblock rows, shared by every image
+  uint32_t stage_bytes;   // bytes of the staging area this run uses
+  uint32_t pad0;
+  uint64_t pad[2];
 };
+static_assert(sizeof(BatchRun) == 64, "run words fill one 64-byte line");
 
 struct QueueArgs {
+  const BatchRun* run;           // sizes of the run (device copy)
   const BatchImage* images;
   const long long* cb_offset;    // first global codeblock of each image (n_images + 1 entries)
   const long long* sorted_base;  // first entry of each image in `sorted` (even-padded ranges)
-  int n_images;
-  long long total_cbs;
-  long long total_pairs;
-  int rows;
   int decode;
+  int max_images;                // capacity of the batch object (grid of the per-image scan)
+  int cus;                       // compute units (grids of the grid-stride kernels)
   uint32_t* keys;        // per global codeblock: MSB bucket
   uint32_t* sorted;      // 2*total_pairs entries: image << 20 | codeblock, sorted by MSB inside each image
   uint32_t* pair_keys;   // per pair: pass mode * 29 + bucket
@@ -82,10 +93,20 @@ struct BatchArgs {
   const uint32_t* pair_keys;  // per pair: pass mode * 29 + MSB bucket of its costlier codeblock
   uint32_t* wide;             // pairs deferred to the 28-plane pass
   uint32_t* ctl;
-  long long n_pairs;
-  int rows;
+  const BatchRun* run;  // total_pairs and rows of the run
   int wide_pass;  // 0: drain the pair queue (deferring pairs beyond the kernel's planes); 1: drain `wide`
+  int fold;         // this launch ends the run: its last wave folds the run's status
+  int report_wide;  // the 28-plane pass is left out: the fold reports deferred pairs
 };
+
+// Batched run (bpc_batch_run) with two more outputs for the server: host_status
+// (nullable, pinned host memory) receives the run's ST_* bits from the last kernel
+// of the run, and defer_wide != 0 leaves the 28-plane pass out of the run,
+// reporting ST_WIDE_PENDING in host_status when pairs wait for it; batch_run_wide
+// then runs it (on the batch's last run, before the batch object runs again).
+int batch_run_ex(bpc_batch* b, int direction, const bpc_image* images, int n_images, int use_graph,
+                 uint32_t* d_status, uint32_t* host_status, int defer_wide, void* stream);
+int batch_run_wide(bpc_batch* b, void* stream);
 
 // Symbol statistics of a training image (LUT trainer).
 struct TrainArgs {
@@ -116,7 +137,10 @@ hipError_t launch_queue_build(const QueueArgs& a, hipStream_t stream);
 // vgpr_cap_waves: 0 (compiler's register budget) or 5..8 (registers capped for that many waves per SIMD)
 hipError_t launch_batch(int decode, int maxp, const BatchArgs& a, int grid_blocks, int waves_per_block,
                         int vgpr_cap_waves, hipStream_t stream);
-hipError_t launch_status_fold(uint32_t* ctl, const BatchRun* run, hipStream_t stream);
+// Copies the staging area of a run (stage_bytes, from its run words) from pinned
+// host memory to the device and clears the queue's control words.
+hipError_t launch_batch_stage(const char* h_stage, char* d_stage, uint32_t* ctl, hipStream_t stream);
+
 // Workgroups of the batched kernel that fit on one CU (occupancy query).
 int batch_resident_blocks_per_cu(int decode, int maxp, int waves_per_block, int vgpr_cap_waves);
 // Trainer workgroups that fit on one CU with lds_bytes of LDS each.

@@ -1052,6 +1052,33 @@ __global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
 // Codeblocks are sorted by MSB inside each image and paired there (both halves of
 // a wave always code the same image, so its descriptor is wave-uniform); pairs
 // are then ordered by (pass mode, MSB) with the costliest first.
+// Counts the calling wave out of its launch.  When the launch ends the run
+// (a.fold), the grid's last wave folds the run's status: an atomic OR of its bits
+// into the caller's device word and a store of them into the host-visible word,
+// with ST_WIDE_PENDING when deferred pairs wait for a 28-plane pass that the run
+// left out.  No trailing kernel: under concurrent runs a small dispatch after the
+// main pass waits for a CU behind other runs' persistent kernels.
+__device__ __forceinline__ void batch_wave_done(const BatchArgs& a, bool wide_pass) {
+  if (!a.fold) return;
+  const int lane = threadIdx.x & 63;
+  u32 done = 0;
+  if (lane == 0) {
+    __threadfence();  // the wave's status and deferral atomics come before its count
+    done = atomicAdd(&a.ctl[wide_pass ? CTL_DONE_WIDE : CTL_DONE], 1u);
+  }
+  done = (u32)__builtin_amdgcn_readfirstlane((int)done);
+  if (done != gridDim.x * (blockDim.x >> 6) - 1u || lane != 0) return;
+  __threadfence();
+  const u32 bits = atomicOr(&a.ctl[CTL_STATUS], 0u);
+  const BatchRun* run = a.run;
+  if (bits && run->status) atomicOr(run->status, bits);
+  if (run->host_status) {
+    const u32 pending = a.report_wide && atomicOr(&a.ctl[CTL_WIDE_COUNT], 0u) ? ST_WIDE_PENDING : 0u;
+    *reinterpret_cast<volatile u32*>(run->host_status) = bits | pending;
+    __threadfence_system();
+  }
+}
+
 // Persistent kernel: every wave takes the next pair off the queue until it is drained.
 // OCC = 0 leaves the register budget to the compiler (up to 8 waves per SIMD);
 // OCC = 5..8 caps it so that that many waves fit on a SIMD, at the price of
@@ -1059,6 +1086,17 @@ __global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
 template <int DEC, int MAXP, int OCC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC ? OCC : 1, OCC ? OCC : 8)))
 void bpc_batch_kernel(const BatchArgs a) {
+  // The main pass drains the pair queue and defers the pairs whose MSB is beyond
+  // its planes; a 28-plane pass then drains the deferred pairs.  Sizes come from
+  // the run words (one captured sequence serves batches of any shape).
+  const bool wide_pass = a.wide_pass != 0;
+  const u32 n_items =
+      (u32)__builtin_amdgcn_readfirstlane((int)(wide_pass ? a.ctl[CTL_WIDE_COUNT] : (u32)a.run->total_pairs));
+  if (n_items == 0) {  // uniform over the workgroup: before its first barrier
+    batch_wave_done(a, wide_pass);
+    return;
+  }
+  const int rows = __builtin_amdgcn_readfirstlane(a.run->rows);
   __shared__ uint8_t slut[256];
   __shared__ uint32_t prows[4 * 64 * PROW_BYTES / 4];  // per wave of the workgroup (at most 4)
   const lds_u8 sl = build_sign_lut(slut);
@@ -1066,12 +1104,8 @@ void bpc_batch_kernel(const BatchArgs a) {
   const int half = (threadIdx.x & 63) >> 5;
   const size_t wave = (size_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   void* scr = reinterpret_cast<char*>(a.scratch) + wave * (2 * MAXP * 64 * 8);
-  // The main pass drains the pair queue and defers the pairs whose MSB is beyond
-  // its planes; a 28-plane pass then drains the deferred pairs.
-  const bool wide_pass = a.wide_pass != 0;
   const u32* queue = wide_pass ? a.wide : a.pair_queue;
   u32* head = &a.ctl[wide_pass ? CTL_WIDE_HEAD : CTL_HEAD];
-  const u32 n_items = wide_pass ? (u32)__builtin_amdgcn_readfirstlane((int)a.ctl[CTL_WIDE_COUNT]) : (u32)a.n_pairs;
   for (;;) {
     u32 slot = 0;
     if ((threadIdx.x & 63) == 0) slot = atomicAdd(head, 1u);
@@ -1108,10 +1142,11 @@ void bpc_batch_kernel(const BatchArgs a) {
     d.prows = pr;
     d.cb = e == QUEUE_EMPTY ? -1 : (int)queue_entry_cb(e);
     if (im.cpass3)
-      code_pair<DEC, 1, MAXP>(d, a.rows, &a.ctl[CTL_STATUS], scr);
+      code_pair<DEC, 1, MAXP>(d, rows, &a.ctl[CTL_STATUS], scr);
     else
-      code_pair<DEC, 0, MAXP>(d, a.rows, &a.ctl[CTL_STATUS], scr);
+      code_pair<DEC, 0, MAXP>(d, rows, &a.ctl[CTL_STATUS], scr);
   }
+  batch_wave_done(a, wide_pass);
 }
 
 // ---- LUT trainer ------------------------------------------------------------

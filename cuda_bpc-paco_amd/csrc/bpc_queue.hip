@@ -37,10 +37,19 @@ __device__ __forceinline__ u32 wave_or32(u32 v) {
   return (u32)__builtin_amdgcn_readlane((int)v, 63);
 }
 
-// Clears the queue counters (kernels rather than memsets, so captured graphs
-// hold no memset nodes).
-__global__ void queue_reset_kernel(u32* ctl, int words) {
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < words; k += gridDim.x * blockDim.x) ctl[k] = 0;
+// Copies the run's staging area (its size is in the run words at its head) from
+// pinned host memory into device memory, 16 bytes per thread, and clears the
+// queue's control words (kernels rather than a copy and a memset: the copy is a
+// read of host memory by the first kernel of the run, and captured graphs hold no
+// memset nodes).
+__global__ __launch_bounds__(256) void batch_stage_kernel(const char* h_stage, char* d_stage, u32* ctl) {
+  const BatchRun* hr = reinterpret_cast<const BatchRun*>(h_stage);
+  const u32 bytes = hr->stage_bytes;
+  const int words = CTL_IMG + 32 * hr->n_images;
+  const u32 tid = blockIdx.x * blockDim.x + threadIdx.x, nthreads = gridDim.x * blockDim.x;
+  for (u32 k = tid; k < bytes / 16u; k += nthreads)
+    reinterpret_cast<uint4*>(d_stage)[k] = reinterpret_cast<const uint4*>(h_stage)[k];
+  for (int k = (int)tid; k < words; k += (int)nthreads) ctl[k] = 0;
 }
 
 // Adds 1 to counter[key] for every active lane, with one atomic per distinct key in
@@ -67,22 +76,32 @@ __device__ __forceinline__ u32 wave_atomic_inc(u32* counter, u32 key) {
 
 __device__ __forceinline__ u32 mag(int v) { return v < 0 ? 0u - (u32)v : (u32)v; }
 
-// Encoder keys: one wave per codeblock ORs its magnitudes; MSB bucket, counted per image.
-__global__ __launch_bounds__(256) void queue_keys_kernel(const QueueArgs a) {
-  const long long gcb = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (gcb >= a.total_cbs) return;  // wave-uniform
-  int lo = 0, hi = a.n_images - 1;  // image owning gcb: last k with cb_offset[k] <= gcb
+// Image owning global codeblock gcb: the last k with cb_offset[k] <= gcb.
+__device__ __forceinline__ int image_of(const long long* cb_offset, int n_images, long long gcb) {
+  int lo = 0, hi = n_images - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (a.cb_offset[mid] <= gcb) lo = mid; else hi = mid - 1;
+    if (cb_offset[mid] <= gcb) lo = mid; else hi = mid - 1;
   }
-  const BatchImage& im = a.images[lo];
-  const int cb = (int)(gcb - a.cb_offset[lo]);
-  int msb;
-  {
+  return lo;
+}
+
+// Grid-stride index of a thread and the thread count of the grid.
+__device__ __forceinline__ long long thread_index() { return (long long)blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ __forceinline__ long long grid_threads() { return (long long)gridDim.x * blockDim.x; }
+
+// Encoder keys: one wave per codeblock ORs its magnitudes; MSB bucket, counted per image.
+__global__ __launch_bounds__(256) void queue_keys_kernel(const QueueArgs a) {
+  const long long total = a.run->total_cbs;
+  const int n_images = a.run->n_images, rows = a.run->rows;
+  const int lane = threadIdx.x & 63;
+  const long long nwaves = grid_threads() >> 6;
+  for (long long gcb = thread_index() >> 6; gcb < total; gcb += nwaves) {  // wave-uniform
+    const int img = image_of(a.cb_offset, n_images, gcb);
+    const BatchImage& im = a.images[img];
+    const int cb = (int)(gcb - a.cb_offset[img]);
     const int tx = cb % im.nx, ty = cb / im.nx;
-    const int64_t base = im.layout ? (int64_t)cb * 64 * a.rows : (int64_t)ty * a.rows * im.size_x + tx * 64;
+    const int64_t base = im.layout ? (int64_t)cb * 64 * rows : (int64_t)ty * rows * im.size_x + tx * 64;
     const int rs = im.layout ? 64 : im.size_x;
     const int32_t* p0 = im.src + base;
     u32 m = 0;
@@ -90,75 +109,73 @@ __global__ __launch_bounds__(256) void queue_keys_kernel(const QueueArgs a) {
       // 16 bytes per lane: four rows of 64 coefficients per wave load, loads in flight
       const int col = (lane & 15) * 4;
 #pragma unroll 4
-      for (int r = lane >> 4; r < a.rows; r += 4) {
+      for (int r = lane >> 4; r < rows; r += 4) {
         const int4 v = *reinterpret_cast<const int4*>(p0 + (int64_t)r * rs + col);
         m |= mag(v.x) | mag(v.y) | mag(v.z) | mag(v.w);
       }
     } else {
-      for (int r = 0; r < a.rows; ++r) m |= mag(p0[(int64_t)r * rs + lane]);
+      for (int r = 0; r < rows; ++r) m |= mag(p0[(int64_t)r * rs + lane]);
     }
     m = wave_or32(m);
-    msb = m ? 31 - __clz(m) : 32;
-  }
-  if (lane == 0) {
-    const u32 b = msb_bucket(msb);
-    a.keys[gcb] = b;
-    atomicAdd(&a.ctl[CTL_IMG + lo * 32 + b], 1u);
+    const int msb = m ? 31 - __clz(m) : 32;
+    if (lane == 0) {
+      const u32 b = msb_bucket(msb);
+      a.keys[gcb] = b;
+      atomicAdd(&a.ctl[CTL_IMG + img * 32 + b], 1u);
+    }
   }
 }
 
 // Decoder keys: one thread per codeblock reads the MSB word of its region.
 __global__ __launch_bounds__(256) void queue_keys_decode_kernel(const QueueArgs a) {
-  const long long gcb = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gcb >= a.total_cbs) return;
-  int lo = 0, hi = a.n_images - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (a.cb_offset[mid] <= gcb) lo = mid; else hi = mid - 1;
+  const long long total = a.run->total_cbs;
+  const int n_images = a.run->n_images;
+  for (long long gcb = thread_index(); gcb < total; gcb += grid_threads()) {
+    const int img = image_of(a.cb_offset, n_images, gcb);
+    const BatchImage& im = a.images[img];
+    const u32 b = msb_bucket(im.src[(gcb - a.cb_offset[img]) * im.stride]);
+    a.keys[gcb] = b;
+    (void)wave_atomic_inc(a.ctl + CTL_IMG, (u32)img * 32u + b);
   }
-  const BatchImage& im = a.images[lo];
-  const u32 b = msb_bucket(im.src[(gcb - a.cb_offset[lo]) * im.stride]);
-  a.keys[gcb] = b;
-  (void)wave_atomic_inc(a.ctl + CTL_IMG, (u32)lo * 32u + b);
 }
 
 // One thread per image: bucket cursors inside the image's (even-padded) range.
 __global__ void queue_image_scan_kernel(const QueueArgs a) {
-  const int img = blockIdx.x * blockDim.x + threadIdx.x;
-  if (img >= a.n_images) return;
-  u32 pos = (u32)a.sorted_base[img];
-  for (int b = 0; b < NBUCKET; ++b) {
-    u32* c = &a.ctl[CTL_IMG + img * 32 + b];
-    const u32 n = *c;
-    *c = pos;
-    pos += n;
+  const int n_images = a.run->n_images;
+  for (int img = (int)thread_index(); img < n_images; img += (int)grid_threads()) {
+    u32 pos = (u32)a.sorted_base[img];
+    for (int b = 0; b < NBUCKET; ++b) {
+      u32* c = &a.ctl[CTL_IMG + img * 32 + b];
+      const u32 n = *c;
+      *c = pos;
+      pos += n;
+    }
+    const long long ncb = a.cb_offset[img + 1] - a.cb_offset[img];
+    if (ncb & 1) a.sorted[pos] = QUEUE_EMPTY;
   }
-  const long long ncb = a.cb_offset[img + 1] - a.cb_offset[img];
-  if (ncb & 1) a.sorted[pos] = QUEUE_EMPTY;
 }
 
 __global__ __launch_bounds__(256) void queue_scatter_kernel(const QueueArgs a) {
-  const long long gcb = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gcb >= a.total_cbs) return;
-  int lo = 0, hi = a.n_images - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (a.cb_offset[mid] <= gcb) lo = mid; else hi = mid - 1;
+  const long long total = a.run->total_cbs;
+  const int n_images = a.run->n_images;
+  for (long long gcb = thread_index(); gcb < total; gcb += grid_threads()) {
+    const int img = image_of(a.cb_offset, n_images, gcb);
+    const u32 pos = wave_atomic_inc(a.ctl + CTL_IMG, (u32)img * 32u + a.keys[gcb]);
+    a.sorted[pos] = queue_entry((u32)img, (u32)(gcb - a.cb_offset[img]));
   }
-  const u32 pos = wave_atomic_inc(a.ctl + CTL_IMG, (u32)lo * 32u + a.keys[gcb]);
-  a.sorted[pos] = queue_entry((u32)lo, (u32)(gcb - a.cb_offset[lo]));
 }
 
 // One thread per pair: key (pass mode, bucket of its costlier codeblock).
 __global__ __launch_bounds__(256) void queue_pair_keys_kernel(const QueueArgs a) {
-  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.total_pairs) return;
-  const u32 e = a.sorted[2 * p];
-  const int img = (int)queue_entry_image(e);
-  const u32 b = a.keys[a.cb_offset[img] + queue_entry_cb(e)];
-  const u32 k = (a.images[img].cpass3 ? NBUCKET : 0) + b;
-  a.pair_keys[p] = k;
-  (void)wave_atomic_inc(a.ctl + CTL_PAIRHIST, k);
+  const long long total = a.run->total_pairs;
+  for (long long p = thread_index(); p < total; p += grid_threads()) {
+    const u32 e = a.sorted[2 * p];
+    const int img = (int)queue_entry_image(e);
+    const u32 b = a.keys[a.cb_offset[img] + queue_entry_cb(e)];
+    const u32 k = (a.images[img].cpass3 ? NBUCKET : 0) + b;
+    a.pair_keys[p] = k;
+    (void)wave_atomic_inc(a.ctl + CTL_PAIRHIST, k);
+  }
 }
 
 __global__ void queue_pair_scan_kernel(const QueueArgs a) {
@@ -172,42 +189,35 @@ __global__ void queue_pair_scan_kernel(const QueueArgs a) {
 }
 
 __global__ __launch_bounds__(256) void queue_pair_scatter_kernel(const QueueArgs a) {
-  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.total_pairs) return;
-  const u32 pos = wave_atomic_inc(a.ctl + CTL_PAIRHIST, a.pair_keys[p]);
-  a.pair_queue[pos] = (u32)p;
-}
-
-// Folds the batch's status into the caller's status word (named by the run's
-// uploaded words, so that a replayed graph folds into the current caller's word).
-__global__ void status_fold_kernel(u32* ctl, const BatchRun* run) {
-  if (threadIdx.x == 0 && ctl[CTL_STATUS]) atomicOr(run->status, ctl[CTL_STATUS]);
+  const long long total = a.run->total_pairs;
+  for (long long p = thread_index(); p < total; p += grid_threads()) {
+    const u32 pos = wave_atomic_inc(a.ctl + CTL_PAIRHIST, a.pair_keys[p]);
+    a.pair_queue[pos] = (u32)p;
+  }
 }
 
 }  // namespace
 
+// Grids of the grid-stride kernels depend on the device only (not on the run), so
+// that a captured sequence replays for any batch: 32 waves per CU (a full CU) for
+// the one-wave-per-codeblock encoder keys, which stream the whole input, and 4
+// workgroups of 256 threads per CU for the one-thread-per-item kernels.
 hipError_t launch_queue_build(const QueueArgs& a, hipStream_t stream) {
-  const int words = CTL_IMG + 32 * a.n_images;
-  hipLaunchKernelGGL(queue_reset_kernel, dim3((words + 255) / 256), dim3(256), 0, stream, a.ctl, words);
-  if (a.total_cbs > 0) {
-    if (a.decode)
-      hipLaunchKernelGGL(queue_keys_decode_kernel, dim3((unsigned)((a.total_cbs + 255) / 256)), dim3(256), 0, stream,
-                         a);
-    else
-      hipLaunchKernelGGL(queue_keys_kernel, dim3((unsigned)((a.total_cbs + 3) / 4)), dim3(256), 0, stream, a);
-    hipLaunchKernelGGL(queue_image_scan_kernel, dim3((a.n_images + 63) / 64), dim3(64), 0, stream, a);
-    hipLaunchKernelGGL(queue_scatter_kernel, dim3((unsigned)((a.total_cbs + 255) / 256)), dim3(256), 0, stream, a);
-    hipLaunchKernelGGL(queue_pair_keys_kernel, dim3((unsigned)((a.total_pairs + 255) / 256)), dim3(256), 0, stream, a);
-    hipLaunchKernelGGL(queue_pair_scan_kernel, dim3(1), dim3(64), 0, stream, a);
-    hipLaunchKernelGGL(queue_pair_scatter_kernel, dim3((unsigned)((a.total_pairs + 255) / 256)), dim3(256), 0, stream,
-                       a);
-  }
+  const unsigned wave_grid = (unsigned)a.cus * 8, thread_grid = (unsigned)a.cus * 4;
+  if (a.decode)
+    hipLaunchKernelGGL(queue_keys_decode_kernel, dim3(thread_grid), dim3(256), 0, stream, a);
+  else
+    hipLaunchKernelGGL(queue_keys_kernel, dim3(wave_grid), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL(queue_image_scan_kernel, dim3((a.max_images + 63) / 64), dim3(64), 0, stream, a);
+  hipLaunchKernelGGL(queue_scatter_kernel, dim3(thread_grid), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL(queue_pair_keys_kernel, dim3(thread_grid), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL(queue_pair_scan_kernel, dim3(1), dim3(64), 0, stream, a);
+  hipLaunchKernelGGL(queue_pair_scatter_kernel, dim3(thread_grid), dim3(256), 0, stream, a);
   return hipGetLastError();
 }
 
-hipError_t launch_status_fold(uint32_t* ctl, const BatchRun* run, hipStream_t stream) {
-  hipLaunchKernelGGL(status_fold_kernel, dim3(1), dim3(64), 0, stream, ctl, run);
+hipError_t launch_batch_stage(const char* h_stage, char* d_stage, uint32_t* ctl, hipStream_t stream) {
+  hipLaunchKernelGGL(batch_stage_kernel, dim3(64), dim3(256), 0, stream, h_stage, d_stage, ctl);
   return hipGetLastError();
 }
-
 }  // namespace bpc

@@ -23,6 +23,7 @@
 
 #include "../../include/bpc_paco.h"
 #include "bpc_host.h"
+#include "bpc_internal.h"
 
 namespace {
 
@@ -40,11 +41,17 @@ struct Req {
   bool done = false;
 };
 
+// The status word a batch's last kernel has not written yet (no ST_* combination).
+constexpr uint32_t kStatusUnset = 0x40000000u;
+
 struct Slot {
   bpc_batch* batch = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t ev = nullptr;
-  uint32_t* d_status = nullptr;
+  // the batch's status bits, stored by its last kernel into pinned host memory: read
+  // once the event has completed, with no copy on the device
+  volatile uint32_t* h_status = nullptr;
+  uint32_t* h_status_dev = nullptr;  // its device address
   std::vector<Req*> reqs;
   bool busy = false;
 };
@@ -82,8 +89,7 @@ struct bpc_server {
 
 namespace {
 
-void finish_slot(bpc_server* s, Slot& sl) {
-  const int st = bpc_status_take(sl.d_status, sl.stream);  // the batch is done: no wait
+void finish_slot(bpc_server* s, Slot& sl, int st) {
   const auto now = Clock::now();
   std::lock_guard<std::mutex> lk(s->mu);
   for (Req* q : sl.reqs) {
@@ -96,6 +102,26 @@ void finish_slot(bpc_server* s, Slot& sl) {
   sl.reqs.clear();
   sl.busy = false;
   s->cv_done.notify_all();
+}
+
+// The slot's event has completed: either the batch is done, or its main pass
+// deferred pairs to the 28-plane pass, which is launched now (the slot stays busy).
+void complete_slot(bpc_server* s, Slot& sl) {
+  std::atomic_thread_fence(std::memory_order_acquire);
+  const uint32_t bits = *sl.h_status;
+  if (bits == kStatusUnset) {
+    finish_slot(s, sl, bpc::fail(BPC_ERR_HIP, "server: the batch's status word was not written"));
+    return;
+  }
+  if (bits & bpc::ST_WIDE_PENDING) {
+    *sl.h_status = kStatusUnset;
+    int st = bpc::batch_run_wide(sl.batch, sl.stream);
+    if (!st && hipEventRecord(sl.ev, sl.stream) != hipSuccess) st = bpc::fail(BPC_ERR_HIP, "server: event record");
+    if (!st) return;
+    finish_slot(s, sl, st);
+    return;
+  }
+  finish_slot(s, sl, bpc::status_to_code(bits));
 }
 
 size_t ncw_bytes(const bpc_request& r) { return sizeof(int32_t) * (size_t)std::max<int64_t>(1, ncb_of(r)); }
@@ -222,8 +248,12 @@ int launch_batch(bpc_server* s, Slot& sl, std::vector<Req*>& reqs) {
     im.params = q->r.params;
     im.fill_unused = 1;
   }
-  int st = bpc_batch_run(sl.batch, reqs.front()->r.direction, imgs.data(), (int)imgs.size(), 0, sl.d_status,
-                         sl.stream);
+  // Replayed graph (one per direction and bitplane count: the run's shape is read
+  // on the device), status into the slot's pinned word, 28-plane pass only when
+  // the main pass deferred pairs to it.
+  *sl.h_status = kStatusUnset;
+  int st = bpc::batch_run_ex(sl.batch, reqs.front()->r.direction, imgs.data(), (int)imgs.size(), 1, nullptr,
+                             sl.h_status_dev, 1, sl.stream);
   if (st) return st;
   if (hipEventRecord(sl.ev, sl.stream) != hipSuccess) return BPC_ERR_HIP;
   sl.reqs = reqs;
@@ -237,7 +267,7 @@ void scheduler(bpc_server* s) {
     bool progressed = false;
     for (Slot& sl : s->slots) {  // completions
       if (sl.busy && hipEventQuery(sl.ev) == hipSuccess) {
-        finish_slot(s, sl);
+        complete_slot(s, sl);
         progressed = true;
       }
     }
@@ -280,9 +310,9 @@ void scheduler(bpc_server* s) {
     }
   }
   for (Slot& sl : s->slots)
-    if (sl.busy) {
+    while (sl.busy) {
       (void)hipEventSynchronize(sl.ev);
-      finish_slot(s, sl);
+      complete_slot(s, sl);
     }
 }
 
@@ -337,8 +367,12 @@ int bpc_server_create(bpc_server** out, const bpc_server_config* cfg) {
     if (!st) st = bpc_batch_set_vgpr_cap(sl.batch, c.vgpr_cap_waves);
     if (!st && hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess) st = BPC_ERR_HIP;
     if (!st && hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess) st = BPC_ERR_HIP;
-    if (!st && hipMalloc(&sl.d_status, sizeof(uint32_t)) != hipSuccess) st = BPC_ERR_HIP;
-    if (!st && hipMemset(sl.d_status, 0, sizeof(uint32_t)) != hipSuccess) st = BPC_ERR_HIP;
+    void* hs = nullptr;
+    if (!st && hipHostMalloc(&hs, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) st = BPC_ERR_HIP;
+    sl.h_status = static_cast<volatile uint32_t*>(hs);
+    void* hd = nullptr;
+    if (!st && hipHostGetDevicePointer(&hd, hs, 0) != hipSuccess) st = BPC_ERR_HIP;
+    sl.h_status_dev = static_cast<uint32_t*>(hd);
     if (st) {
       s->stop = true;
       bpc_server_destroy(s);
@@ -446,7 +480,7 @@ int bpc_server_destroy(bpc_server* s) {
     if (sl.batch) bpc_batch_destroy(sl.batch);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
     if (sl.ev) (void)hipEventDestroy(sl.ev);
-    if (sl.d_status) (void)hipFree(sl.d_status);
+    if (sl.h_status) (void)hipHostFree(const_cast<uint32_t*>(sl.h_status));
   }
   delete s;
   return BPC_OK;

@@ -218,37 +218,49 @@ def test_batch_wide_codeblocks_vs_oracle(bpc, O):
         free_set(items)
 
 
-def test_batch_graph_key_tells_codeblock_lengths_apart(bpc, O):
-    """Two batches with the same image sizes and the same explicit region size but
-    64- and 32-row codeblocks, on a grid held at its cap (1 wave per workgroup, 1
-    wave per SIMD), replayed from graphs: each must come out as the oracle's."""
-    batch = bpc.Batch(2, 1 << 14)
+def test_batch_one_graph_serves_every_batch_shape(bpc, O):
+    """The captured sequence reads the run's shape on the device: batches of
+    different image counts, sizes, pass modes and codeblock lengths (64 and 32
+    rows) replay one graph per direction, on a grid held at its cap (1 wave per
+    workgroup, 1 wave per SIMD), and each comes out as the oracle's."""
+    batch = bpc.Batch(4, 1 << 14)
     batch.set_tuning(1, 1)
+    shapes = [
+        [(4096, 1, 15, 1, 64)],
+        [(1024, 0, 15, 1, 32), (512, 1, 15, 1, 32), (1536, 0, 15, 1, 32)],
+        [(512, 0, 15, 1, 64), (2048, 1, 15, 1, 64)],
+        [(4096, 0, 15, 1, 32)],
+    ]
     try:
-        for rows in (64, 32, 64, 32):
-            items = make_set(bpc, [(4096, 1, 15, 1, rows)], seed=8000, stride=4096)
+        for k, specs in enumerate(shapes):
+            items = make_set(bpc, specs, seed=8000 + 10 * k, stride=4096)
             try:
                 batch_encode_decode(bpc, batch, items, use_graph=True)
                 check_oracle(O, items)
             finally:
                 free_set(items)
-        assert batch.graph_count() == 4  # (encode, decode) x (64, 32 rows)
+            assert batch.graph_count() == 2  # encode, decode
     finally:
         batch.destroy()
 
 
 def test_batch_graph_cache_is_bounded(bpc):
-    """At most 32 captured graphs per batch object; the least recently used go."""
+    """At most 32 captured graphs per batch object (one per direction, bitplane
+    count and launch tuning); the least recently used go, and a batch replayed
+    after its graph was dropped is captured again."""
     batch = bpc.Batch(2, 1 << 12)
-    items = make_set(bpc, [(64 * k, 0, 15, 1, 64) for k in range(2, 38)], seed=8500)
+    items = make_set(bpc, [(256, 0, 15, 1, 64)], seed=8500)
+    it = items[0]
+    tunings = [(wpb, wps) for wps in range(0, 9) for wpb in (1, 2, 4)]  # 27 x 2 directions
     try:
-        for it in items:
+        for wpb, wps in tunings + tunings[:3]:
+            batch.set_tuning(wpb, wps)
+            it["d_out"].fill_bytes(0x55)
             batch_encode_decode(bpc, batch, [it], use_graph=True)
             assert batch.graph_count() <= 32
-        assert batch.graph_count() == 32
-        for it in items:
             got = synth.from_tiled(it["d_out"].download(), it["size"], it["size"])
-            assert np.array_equal(got, it["img"])
+            assert np.array_equal(got, it["img"]), (wpb, wps)
+        assert batch.graph_count() == 32
     finally:
         batch.destroy()
         free_set(items)
