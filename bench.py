@@ -794,7 +794,8 @@ def main():
                          "workload's autotuner overrides it when both run)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB 5 times (HBM counter calibration)")
-    ap.add_argument("--served-requests", type=int, default=600, help="requests per offered rate")
+    ap.add_argument("--served-requests", type=int, default=2000,
+                    help="requests per offered rate (the drain of the last batch is a smaller share of a longer run)")
     ap.add_argument("--detail-out", default=os.path.join(ROOT, "gpurun_out", "bench_detail.json"),
                     help="JSON file for the autotuner and served-rate tables ('' = none)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")

@@ -73,6 +73,7 @@ class Image(ctypes.Structure):
         ("size_y", ctypes.c_int32),
         ("params", Params),
         ("fill_unused", ctypes.c_int32),
+        ("int16_coeffs", ctypes.c_int32),
     ]
 
 
@@ -92,7 +93,7 @@ class ServerConfig(ctypes.Structure):
 class Request(ctypes.Structure):
     _fields_ = [("direction", ctypes.c_int32), ("src", ctypes.c_void_p), ("lut", ctypes.c_void_p),
                 ("lut_len", ctypes.c_int32), ("size_x", ctypes.c_int32), ("size_y", ctypes.c_int32),
-                ("params", Params)]
+                ("params", Params), ("int16_coeffs", ctypes.c_int32)]
 
 
 class Result(ctypes.Structure):
@@ -151,6 +152,7 @@ def lib():
             "bpc_decode_device": (i32, [P, P, i32, i32, P, i32, PP, P, P]),
             "bpc_encode_device_i16": (i32, [P, P, i32, i32, P, i32, PP, i32, P, P, P]),
             "bpc_decode_device_i16": (i32, [P, P, i32, i32, P, i32, PP, P, P]),
+            "bpc_decode_compact_device": (i32, [P, P, P, ctypes.c_uint64, P, i32, i32, P, i32, PP, P, P]),
             "bpc_status_take": (i32, [P, P]),
             "bpc_codec_device_noalloc": (i32, [i32, P, P, i32, i32, P, i32, PP, i32, P, P, P]),
             "bpc_dev_malloc": (i32, [ctypes.POINTER(P), sz]),
@@ -556,10 +558,11 @@ class Batch:
             pass
 
 
-def make_image(src, dst, lut, size_x, size_y, p, ncw=None, fill_unused=1):
-    """Image descriptor from DeviceBuffers and a Lut."""
+def make_image(src, dst, lut, size_x, size_y, p, ncw=None, fill_unused=1, int16=False):
+    """Image descriptor from DeviceBuffers and a Lut; int16: the coefficients (encoder
+    src, decoder dst) are int16."""
     return Image(src.ptr, dst.ptr, lut.ptr, lut.len, ncw.ptr if ncw is not None else None, size_x, size_y, p,
-                 fill_unused)
+                 fill_unused, 1 if int16 else 0)
 
 
 # ---- continuous-batching server ------------------------------------------------------
@@ -597,8 +600,9 @@ class Server:
         _check(lib().bpc_server_create(ctypes.byref(p), ctypes.byref(cfg)))
         self.ptr = p
 
-    def submit(self, direction, src, lut, size_x, size_y, p):
-        req = Request(direction, src.ptr if hasattr(src, "ptr") else src, lut.ptr, lut.len, size_x, size_y, p)
+    def submit(self, direction, src, lut, size_x, size_y, p, int16=False):
+        req = Request(direction, src.ptr if hasattr(src, "ptr") else src, lut.ptr, lut.len, size_x, size_y, p,
+                      1 if int16 else 0)
         rid = ctypes.c_uint64()
         _check(lib().bpc_server_submit(self.ptr, ctypes.byref(req), ctypes.byref(rid)))
         return rid.value
@@ -684,6 +688,13 @@ def compact_unpack(c, d_stream, stride, d_status=None, stream=None):
     """Compact -> reference-layout regions (MSB, codewords, -1 fill) in d_stream."""
     _check(lib().bpc_compact_unpack_device(_dptr(c.msb), _dptr(c.offsets), _dptr(c.payload), c.total, c.ncb, stride,
                                            _dptr(d_stream), _dptr(d_status), _sp(stream)))
+
+
+def decode_compact_device(c, d_coeffs, size_x, size_y, lut, p, d_status=None, stream=None):
+    """Compact -> int32 coefficients in p.layout, with no regions in between."""
+    _check(lib().bpc_decode_compact_device(_dptr(c.msb), _dptr(c.offsets), _dptr(c.payload), c.total,
+                                           _dptr(d_coeffs), size_x, size_y, lut.ptr, lut.len, ctypes.byref(p),
+                                           _dptr(d_status), _sp(stream)))
 
 
 def cb_file_order(size_x, size_y, dwt_levels=5, cblock_length=64):

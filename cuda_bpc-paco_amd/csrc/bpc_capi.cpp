@@ -73,14 +73,14 @@ int device_cus(int dev) {
 
 // Resident waves per CU of a per-image kernel variant (an occupancy query, asked
 // once per device and variant).
-int resident_waves_cached(int dev, int decode, int cpass3, int maxp, int reg_planes) {
+int resident_waves_cached(int dev, int decode, int cpass3, int maxp, int reg_planes, int compact) {
   static std::mutex mu;
-  static std::map<std::tuple<int, int, int, int, int>, int> cache;
-  const auto key = std::make_tuple(dev, decode, cpass3, maxp, reg_planes);
+  static std::map<std::tuple<int, int, int, int, int, int>, int> cache;
+  const auto key = std::make_tuple(dev, decode, cpass3, maxp, reg_planes, compact);
   std::lock_guard<std::mutex> lk(mu);
   auto it = cache.find(key);
   if (it != cache.end()) return it->second;
-  int per_cu = bpc::codec_resident_waves_per_cu(decode, cpass3, maxp, reg_planes);
+  int per_cu = bpc::codec_resident_waves_per_cu(decode, cpass3, maxp, reg_planes, compact);
   if (per_cu <= 0) per_cu = 1;
   cache[key] = per_cu;
   return per_cu;
@@ -127,12 +127,21 @@ int scratch_release(int dev, hipStream_t stream) {
   return BPC_OK;
 }
 
+// The decoder's codestream in the compact container (MSB bytes, offsets, payload).
+struct CompactIn {
+  const uint8_t* msb = nullptr;
+  const uint64_t* offsets = nullptr;
+  const uint16_t* payload = nullptr;
+  uint64_t payload_len = 0;
+};
+
 // fixed_scratch (nullable): bitplane slots of fixed_bytes to use instead of the
 // library's scratch; the grid is then limited to the waves whose slots fit.
+// cmp (nullable, decoder): read the compact container instead of src.
 int launch(int decode, const int32_t* src, int32_t* dst, int size_x, int size_y, const int32_t* d_lut, int lut_len,
            const bpc_params* p, int fill, int32_t* d_ncw, uint32_t* d_status, hipStream_t stream, int maxp,
-           int in16 = 0, void* fixed_scratch = nullptr, size_t fixed_bytes = 0) {
-  bpc::KArgs a;
+           int in16 = 0, void* fixed_scratch = nullptr, size_t fixed_bytes = 0, const CompactIn* cmp = nullptr) {
+  bpc::KArgs a{};
   a.src = src;
   a.dst = dst;
   a.lut = d_lut;
@@ -150,6 +159,15 @@ int launch(int decode, const int32_t* src, int32_t* dst, int size_x, int size_y,
   a.layout = p->layout;
   a.fill = fill;
   a.in16 = in16;
+  const int compact = decode && cmp ? 1 : 0;
+  if (compact) {
+    a.cmsb = cmp->msb;
+    a.coff = reinterpret_cast<const unsigned long long*>(cmp->offsets);
+    // a non-null payload selects the compact variant, also for an empty payload
+    static const uint16_t kNoPayload = 0;
+    a.cpay = cmp->payload ? cmp->payload : &kNoPayload;
+    a.ccap = cmp->payload_len;
+  }
   if (a.ncb <= 0) return BPC_OK;
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
@@ -161,13 +179,13 @@ int launch(int decode, const int32_t* src, int32_t* dst, int size_x, int size_y,
   const long long units = a.single ? a.ncb : (a.ncb + 1) / 2;
   // When every wave the image needs is resident at once with the bitplanes held in
   // registers, that variant runs: no scratch slot, no plane traffic to memory.
-  if (maxp <= 20 && units <= (long long)resident_waves_cached(dev, decode, p->cpass3, maxp, 1) * cus) {
+  if (maxp <= 20 && units <= (long long)resident_waves_cached(dev, decode, p->cpass3, maxp, 1, compact) * cus) {
     a.scratch = nullptr;
     hipError_t e = bpc::launch_codec(decode, p->cpass3, maxp, 1, a, (int)units, stream);
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
     return BPC_OK;
   }
-  int per_cu = resident_waves_cached(dev, decode, p->cpass3, maxp, 0);
+  int per_cu = resident_waves_cached(dev, decode, p->cpass3, maxp, 0, compact);
   int grid = (int)std::min<long long>(units, (long long)per_cu * cus);
   if (fixed_scratch) {
     const size_t slots = fixed_bytes / bpc::scratch_bytes_per_wave(maxp);
@@ -257,6 +275,28 @@ int bpc_decode_device_i16(const int32_t* d_stream, int16_t* d_coeffs, int size_x
   }
   return launch(1, d_stream, reinterpret_cast<int32_t*>(d_coeffs), size_x, size_y, d_lut, lut_len, p, 0, nullptr,
                 d_status, (hipStream_t)stream, bpc::planes_for(p->lut_n_bitplanes), 1);
+}
+
+int bpc_decode_compact_device(const uint8_t* d_msb, const uint64_t* d_offsets, const uint16_t* d_payload,
+                              uint64_t payload_len, int32_t* d_coeffs, int size_x, int size_y, const int32_t* d_lut,
+                              int lut_len, const bpc_params* p, uint32_t* d_status, void* stream) {
+  int st = check_device_call(d_offsets, d_coeffs, size_x, size_y, d_lut, lut_len, p);
+  if (st) return st;
+  if (!d_msb) return fail(BPC_ERR_ARG, "null MSB array");
+  if (!d_payload && payload_len) return fail(BPC_ERR_ARG, "null payload of %llu codewords",
+                                             (unsigned long long)payload_len);
+  if ((uintptr_t)d_payload & 1) return fail(BPC_ERR_ARG, "payload must be 2-byte aligned");
+  if (!d_status) {
+    st = default_status(&d_status);
+    if (st) return st;
+  }
+  CompactIn cmp;
+  cmp.msb = d_msb;
+  cmp.offsets = d_offsets;
+  cmp.payload = d_payload;
+  cmp.payload_len = payload_len;
+  return launch(1, nullptr, d_coeffs, size_x, size_y, d_lut, lut_len, p, 0, nullptr, d_status, (hipStream_t)stream,
+                bpc::planes_for(p->lut_n_bitplanes), 0, nullptr, 0, &cmp);
 }
 
 int bpc_codec_device_noalloc(int direction, const int32_t* d_src, int32_t* d_dst, int size_x, int size_y,
@@ -606,7 +646,7 @@ int batch_stage(bpc_batch* b, int direction, const bpc_image* images, int n, Bat
     d.layout = im.params.layout;
     d.fill = im.fill_unused;
     d.cpass3 = im.params.cpass3;
-    d.pad = 0;
+    d.in16 = im.int16_coeffs ? 1 : 0;
     ho[k] = plan.total_cbs;
     ho[b->max_images + 1 + k] = pos;  // sorted_base: a fixed place for the batch object
     plan.total_cbs += ncb;

@@ -111,7 +111,8 @@ __global__ __launch_bounds__(64 * WAVES) void compact_pack_kernel(const int32_t*
   const int lane = threadIdx.x & 63;
   const u64 o = offsets[cb];
   const long long n = (long long)(offsets[cb + 1] - o);
-  if (n < 0 || n > stride - 1 || o + (u64)n > cap) {  // not an index of this stream and payload
+  // not an index of this stream and payload (o + n <= cap, tested without wrapping)
+  if (n < 0 || n > stride - 1 || (u64)n > cap || o > cap - (u64)n) {
     if (lane == 0) atomicOr(status, ST_OVERFLOW);
     return;
   }
@@ -158,7 +159,7 @@ __global__ __launch_bounds__(64 * WAVES) void compact_unpack_kernel(const uint8_
   const int lane = threadIdx.x & 63;
   const u64 o = offsets[cb];
   long long n = (long long)(offsets[cb + 1] - o);
-  if (n < 0 || n > stride - 1 || o + (u64)n > cap) {
+  if (n < 0 || n > stride - 1 || (u64)n > cap || o > cap - (u64)n) {  // o + n <= cap, without wrapping
     if (lane == 0) atomicOr(status, ST_OVERFLOW);
     n = 0;
   }

@@ -29,6 +29,12 @@ struct KArgs {
   int in16;            // coefficients are int16 (encoder input / decoder output), else int32
   void* scratch;       // per wave of the grid: 2 * MAXP * 64 uint2 (scratch_bytes_per_wave); unused
                        // by the variant with the bitplanes in registers
+  // decoder only, all null unless it reads the compact container instead of src:
+  // MSB byte per codeblock, offsets (ncb + 1), payload of ccap codewords
+  const uint8_t* cmsb;
+  const unsigned long long* coff;
+  const uint16_t* cpay;
+  unsigned long long ccap;
 };
 
 // One image of a batch, as the device sees it.
@@ -37,7 +43,8 @@ struct BatchImage {
   int32_t* dst;
   const int32_t* lut;
   int32_t* ncw;
-  int32_t size_x, size_y, nx, ncb, stride, nbp, levels, lut_len, layout, fill, cpass3, pad;
+  int32_t size_x, size_y, nx, ncb, stride, nbp, levels, lut_len, layout, fill, cpass3;
+  int32_t in16;  // coefficients as int16 (encoder input / decoder output), else int32
 };
 
 // Control words of a batch queue (device memory): CTL_IMG + 32*n_images words.
@@ -129,8 +136,9 @@ int planes_for(int nbp);
 // Bitplane scratch of one wave: 2 row halves x MAXP planes x 64 lanes x 8 bytes.
 constexpr size_t scratch_bytes_per_wave(int maxp) { return (size_t)2 * maxp * 64 * 8; }
 // Waves of the per-image kernel that fit on one CU (occupancy query); reg_planes
-// selects the variant that keeps the bitplanes in registers (no scratch).
-int codec_resident_waves_per_cu(int decode, int cpass3, int maxp, int reg_planes);
+// selects the variant that keeps the bitplanes in registers (no scratch), compact
+// the decoder that reads the compact container.
+int codec_resident_waves_per_cu(int decode, int cpass3, int maxp, int reg_planes, int compact);
 hipError_t launch_codec(int decode, int cpass3, int maxp, int reg_planes, const KArgs& a, int grid_waves,
                         hipStream_t stream);
 hipError_t launch_queue_build(const QueueArgs& a, hipStream_t stream);

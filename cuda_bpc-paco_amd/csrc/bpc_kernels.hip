@@ -168,12 +168,25 @@ __device__ __forceinline__ void tally(unsigned long long* hist, int idx, u32 sym
   atomicAdd(hist + idx, (1ull << 32) | (sym ? 0ull : 1ull));
 }
 
+// Where the decoder reads codeword slot k (1 <= k < stride) of its codeblock: the
+// reference's region (word k after the MSB word, BPC_PaCo.cu:1130,1216), or the
+// compact container (payload[offsets[cb] + k - 1] for k up to the codeblock's
+// count, -1 past it: the words a region unpacked from the container holds).
+struct CompactCw {
+  const uint16_t* pay;  // this half's codewords
+  u32 n;                // codewords of this half's codeblock
+};
+__device__ __forceinline__ u32 codeword(cgi32 src, const Geo& g, u32 slot) { return (u32)src[g.region + slot]; }
+__device__ __forceinline__ u32 codeword(const CompactCw& c, const Geo&, u32 slot) {
+  return slot <= c.n ? (u32)c.pay[slot - 1] : 0xFFFFFFFFu;
+}
+
 // Slot reservation for one coder call site (BPC_PaCo.cu:268-277 / :304-316):
 // lanes at the site whose interval is exhausted take consecutive slots in lane
 // order after the half's counter; the counter advances by their number.  The
 // counters stay wave-uniform (SGPRs).
-template <int DEC, int MAXP>
-__device__ __forceinline__ void reserve(LaneState<MAXP>& s, bool at, u64 atm, const Geo& g, cgi32 src) {
+template <int DEC, int MAXP, typename SRC>
+__device__ __forceinline__ void reserve(LaneState<MAXP>& s, bool at, u64 atm, const Geo& g, SRC src) {
   // the call site's lanes whose interval is exhausted.  The encoder has the site's
   // lanes as a wave mask and combines it with one compare on the scalar unit; the
   // decoder has them as a predicate and feeds one compare (0 exactly for the lanes
@@ -193,7 +206,7 @@ __device__ __forceinline__ void reserve(LaneState<MAXP>& s, bool at, u64 atm, co
       s.slot = min(slot, (u32)g.stride - 1u);
       s.L = 0;
       s.Sz = 0xFFFFu;
-      if (DEC) s.cw = (u32)src[g.region + s.slot];
+      if (DEC) s.cw = codeword(src, g, s.slot);
     }
     s.cnt0 = __builtin_amdgcn_readfirstlane(s.cnt0 + nlo);
     s.cnt1 = __builtin_amdgcn_readfirstlane(s.cnt1 + __builtin_popcount(hi));
@@ -327,9 +340,9 @@ __device__ __forceinline__ u32 window(u64 x, int i) {
 // One scan step of the significance pass (CLEAN = 0, SPP_Encoder/SPP_Decoder,
 // BPC_PaCo.cu:412-507) or of the cleanup pass (CLEAN = 1, CP_Encoder/CP_Decoder,
 // BPC_PaCo.cu:510-589) at row i (in word H, window form W) of column ODD.
-template <int ODD, int H, int W, int DEC, int CP3, int CLEAN, int TRAIN, int BF, int MAXP>
+template <int ODD, int H, int W, int DEC, int CP3, int CLEAN, int TRAIN, int BF, int MAXP, typename SRC>
 __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u64 actm, u64 plane, const PlaneLut& pl,
-                                         const Geo& g, cgi32 src, gi32 dst) {
+                                         const Geo& g, SRC src, gi32 dst) {
   constexpr int c = ODD, o = 1 - ODD;
   const int ib = i - 32 * H;
   const u32 bit = 1u << ib;
@@ -457,9 +470,9 @@ __device__ __forceinline__ void sig_step(LaneState<MAXP>& s, int i, bool act, u6
 // "significant without refine flag -> set it" (BPC_PaCo.cu:604) touches only the
 // step's own coefficient, whose flag no later step of the pass reads, so the pass
 // applies it to all coefficients at its end (ref_pass).
-template <int ODD, int H, int DEC, int TRAIN, int BF, int MAXP>
+template <int ODD, int H, int DEC, int TRAIN, int BF, int MAXP, typename SRC>
 __device__ __forceinline__ void ref_step(LaneState<MAXP>& s, int i, bool act, u64 actm, u64 plane, const PlaneLut& pl,
-                                         const Geo& g, cgi32 src, gi32 dst) {
+                                         const Geo& g, SRC src, gi32 dst) {
   constexpr int c = ODD;
   const int ib = i - 32 * H;
   const u64 atm = DEC ? 0 : actm & lanes_ne0(word<H>(s.R[c]) & (1u << ib));
@@ -516,9 +529,9 @@ __device__ __forceinline__ u64 wave_or(u64 x) {
 // are not significant (significance only grows, one coefficient per step); CP
 // only those with the cleanup flag (never set during CP); MRP only those with the
 // refine flag (never set during MRP).  Rows 0-31 and 32-63 run as two loops.
-template <int DEC, int CP3, int CLEAN, int TRAIN, int BF, int MAXP>
+template <int DEC, int CP3, int CLEAN, int TRAIN, int BF, int MAXP, typename SRC>
 __device__ __forceinline__ void sig_pass(LaneState<MAXP>& s, bool act, const u64* plane, const PlaneLut& pl,
-                                         const Geo& g, cgi32 src, gi32 dst) {
+                                         const Geo& g, SRC src, gi32 dst) {
   const u64 rowmask = g.rows >= 64 ? ~0ull : ((1ull << g.rows) - 1ull);
   const u64 actm = DEC ? 0 : ballot(act);  // the encoder's steps take the active lanes as a mask
   const u64 mE = wave_or(act ? (CLEAN ? s.C[0] : (~s.S[0] & rowmask)) : 0ull);
@@ -548,9 +561,9 @@ __device__ __forceinline__ void sig_pass(LaneState<MAXP>& s, bool act, const u64
   }
 }
 
-template <int DEC, int TRAIN, int BF, int MAXP>
+template <int DEC, int TRAIN, int BF, int MAXP, typename SRC>
 __device__ __forceinline__ void ref_pass(LaneState<MAXP>& s, bool act, const u64* plane, const PlaneLut& pl,
-                                         const Geo& g, cgi32 src, gi32 dst) {
+                                         const Geo& g, SRC src, gi32 dst) {
   const u64 actm = DEC ? 0 : ballot(act);  // the encoder's steps take the active lanes as a mask
   const u64 mE = wave_or(act ? s.R[0] : 0ull);  // rows with a refinement symbol in some lane
   const u64 mO = wave_or(act ? s.R[1] : 0ull);
@@ -635,6 +648,12 @@ struct HalfDesc {
   int size_x, size_y, nx, stride, nbp, levels, lut_len, layout, fill;
   int in16;  // coefficients as int16 instead of int32
   int cb;    // codeblock of this half in its image, -1 for an empty half
+  // decoder reading the compact container (code_pair's CMP): MSB bytes, offsets,
+  // payload and its length in codewords
+  const uint8_t* cmsb;
+  const unsigned long long* coff;
+  const uint16_t* cpay;
+  unsigned long long ccap;
   lds_u8 slut;  // sign contexts in LDS (build_sign_lut)
   lds_w8 prows; // this wave's LDS rows, PROW_BYTES per lane (unused in symbol-count mode)
 };
@@ -799,7 +818,9 @@ __device__ __forceinline__ u32 read_planes(cgi32 src32, int64_t cbase, int rs, i
 // unchanged, but every symbol is counted at the LUT entry whose probability would
 // code it (hist, in LDS) instead of being coded; nothing is written to dst.
 // RP = 1: the bitplanes in registers (RegPlanes) instead of the wave's scratch slot.
-template <int DEC, int CP3, int MAXP, int TRAIN = 0, int BF = 0, int RP = 0>
+// CMP = 1 (decoder): the codestream comes from the compact container (MSB byte,
+// codewords at payload[offsets[cb] ...]) instead of the codeblock regions.
+template <int DEC, int CP3, int MAXP, int TRAIN = 0, int BF = 0, int RP = 0, int CMP = 0>
 __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uint32_t* status_word, void* scr_base,
                                           unsigned long long* hist = nullptr) {
   const int lane = threadIdx.x & 63;
@@ -862,6 +883,7 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
   u32 status = 0;
   int msb;
   bool good = valid;
+  CompactCw ccw{nullptr, 0u};  // CMP: this half's codewords
   if (!DEC) {
     // Read_Coefficients (BPC_PaCo.cu:14-44) straight into bit-sliced planes.
     const u32 mor = a.in16 ? read_planes<int16_t>(a.src, cbase, rs, rows, valid, scr, s)
@@ -874,6 +896,24 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
     if (!TRAIN && valid && hl == 0) a.dst[g.region] = msb;  // BPC_PaCo.cu:1155
     if (msb != 32 && msb > 27) {
       if (!TRAIN) status |= ST_RANGE;  // symbol counting skips such a codeblock
+      good = false;
+    }
+  } else if (CMP) {
+    // the compact container's checks of compact_unpack_kernel: a count past the
+    // region or the payload is an overflow and leaves no codewords
+    if (valid) {
+      const unsigned long long o = a.coff[cbc];
+      long long n = (long long)(a.coff[cbc + 1] - o);
+      if (n < 0 || n > a.stride - 1 || (unsigned long long)n > a.ccap || o > a.ccap - (unsigned long long)n) {
+        status |= ST_OVERFLOW;
+        n = 0;
+      }
+      ccw.pay = a.cpay + o;
+      ccw.n = (u32)n;
+    }
+    msb = valid ? (int)a.cmsb[cbc] : 32;  // 255 marks a malformed MSB word
+    if (msb != 32 && (msb < 0 || msb > 27)) {
+      status |= ST_MALFORMED;
       good = false;
     }
   } else {
@@ -945,11 +985,19 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
     if (!DEC && BF && !RP && bp > 0) scr.plane(bp - 1, next);
     if (!DEC && (!BF || RP)) scr.plane(bp, plane);
     const bool act_sr = act && !(CP3 && bp == msb);
-    if (ballot(act_sr)) {
-      sig_pass<DEC, CP3, 0, TRAIN, BF>(s, act_sr, plane, pl, g, a.src, a.dst);
-      ref_pass<DEC, TRAIN, BF>(s, act_sr, plane, pl, g, a.src, a.dst);
+    if constexpr (CMP != 0) {
+      if (ballot(act_sr)) {
+        sig_pass<DEC, CP3, 0, TRAIN, BF>(s, act_sr, plane, pl, g, ccw, a.dst);
+        ref_pass<DEC, TRAIN, BF>(s, act_sr, plane, pl, g, ccw, a.dst);
+      }
+      if (CP3) sig_pass<DEC, CP3, 1, TRAIN, BF>(s, act, plane, pl, g, ccw, a.dst);
+    } else {
+      if (ballot(act_sr)) {
+        sig_pass<DEC, CP3, 0, TRAIN, BF>(s, act_sr, plane, pl, g, a.src, a.dst);
+        ref_pass<DEC, TRAIN, BF>(s, act_sr, plane, pl, g, a.src, a.dst);
+      }
+      if (CP3) sig_pass<DEC, CP3, 1, TRAIN, BF>(s, act, plane, pl, g, a.src, a.dst);
     }
-    if (CP3) sig_pass<DEC, CP3, 1, TRAIN, BF>(s, act, plane, pl, g, a.src, a.dst);
     if (!DEC) {
       if (BF && !RP) {
         plane[0] = next[0];
@@ -1013,8 +1061,9 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
 }
 
 
-// RP = 1: bitplanes in registers (no scratch slot; a.scratch unused).
-template <int DEC, int CP3, int MAXP, int RP>
+// RP = 1: bitplanes in registers (no scratch slot; a.scratch unused).  CMP = 1: the
+// decoder reads the compact container (a.cmsb, a.coff, a.cpay) instead of regions.
+template <int DEC, int CP3, int MAXP, int RP, int CMP = 0>
 __global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
   __shared__ uint8_t slut[256];
   const lds_u8 sl = build_sign_lut(slut);
@@ -1036,6 +1085,10 @@ __global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
   d.layout = a.layout;
   d.fill = a.fill;
   d.in16 = a.in16;
+  d.cmsb = a.cmsb;
+  d.coff = a.coff;
+  d.cpay = a.cpay;
+  d.ccap = a.ccap;
   d.slut = sl;
   d.prows = nullptr;  // the branch-free form keeps its probabilities in registers
   const int half = (threadIdx.x & 63) >> 5;
@@ -1044,7 +1097,7 @@ __global__ __launch_bounds__(64) void bpc_codec_kernel(const KArgs a) {
     const int cb = a.single ? (half ? a.ncb : unit) : 2 * unit + half;
     d.cb = cb < a.ncb ? cb : -1;
     // every wave of this launch may be the critical path: branch-free coder steps
-    code_pair<DEC, CP3, MAXP, 0, 1, RP>(d, a.rows, a.status, scr);
+    code_pair<DEC, CP3, MAXP, 0, 1, RP, CMP>(d, a.rows, a.status, scr);
   }
 }
 
@@ -1137,7 +1190,7 @@ void bpc_batch_kernel(const BatchArgs a) {
     d.lut_len = im.lut_len;
     d.layout = im.layout;
     d.fill = im.fill;
-    d.in16 = 0;
+    d.in16 = im.in16;
     d.slut = sl;
     d.prows = pr;
     d.cb = e == QUEUE_EMPTY ? -1 : (int)queue_entry_cb(e);
@@ -1291,19 +1344,24 @@ int planes_for(int nbp) {
   return 28;
 }
 
+// The compact-container variants exist for the decoder only.
 template <int DEC, int CP3, int MAXP, int RP>
 static hipError_t launch_one(const KArgs& a, int grid_waves, hipStream_t stream) {
-  hipLaunchKernelGGL((bpc_codec_kernel<DEC, CP3, MAXP, RP>), dim3(grid_waves), dim3(64), 0, stream, a);
+  if (DEC && a.cpay)
+    hipLaunchKernelGGL((bpc_codec_kernel<DEC, CP3, MAXP, RP, DEC>), dim3(grid_waves), dim3(64), 0, stream, a);
+  else
+    hipLaunchKernelGGL((bpc_codec_kernel<DEC, CP3, MAXP, RP>), dim3(grid_waves), dim3(64), 0, stream, a);
   return hipGetLastError();
 }
 
 template <int DEC, int CP3, int MAXP, int RP>
-static int plain_resident() {
+static int plain_resident(int compact) {
   int blocks = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, bpc_codec_kernel<DEC, CP3, MAXP, RP>, 64, 0) !=
-      hipSuccess)
-    return 0;
-  return blocks;
+  const hipError_t e =
+      DEC && compact
+          ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, bpc_codec_kernel<DEC, CP3, MAXP, RP, DEC>, 64, 0)
+          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, bpc_codec_kernel<DEC, CP3, MAXP, RP>, 64, 0);
+  return e == hipSuccess ? blocks : 0;
 }
 
 // Register planes exist for 16 and 20 planes (28 would not stay in registers).
@@ -1318,22 +1376,22 @@ static hipError_t launch_p(int maxp, const KArgs& a, int grid_waves, hipStream_t
 }
 
 template <int DEC, int CP3, int RP>
-static int resident_p(int maxp) {
+static int resident_p(int maxp, int compact) {
   switch (maxp) {
-    case 16: return plain_resident<DEC, CP3, 16, RP>();
-    case 20: return plain_resident<DEC, CP3, 20, RP>();
-    case 28: return RP ? 0 : plain_resident<DEC, CP3, 28, 0>();
+    case 16: return plain_resident<DEC, CP3, 16, RP>(compact);
+    case 20: return plain_resident<DEC, CP3, 20, RP>(compact);
+    case 28: return RP ? 0 : plain_resident<DEC, CP3, 28, 0>(compact);
     default: return 0;
   }
 }
 
-int codec_resident_waves_per_cu(int decode, int cpass3, int maxp, int reg_planes) {
+int codec_resident_waves_per_cu(int decode, int cpass3, int maxp, int reg_planes, int compact) {
   if (reg_planes) {
-    if (decode) return cpass3 ? resident_p<1, 1, 1>(maxp) : resident_p<1, 0, 1>(maxp);
-    return cpass3 ? resident_p<0, 1, 1>(maxp) : resident_p<0, 0, 1>(maxp);
+    if (decode) return cpass3 ? resident_p<1, 1, 1>(maxp, compact) : resident_p<1, 0, 1>(maxp, compact);
+    return cpass3 ? resident_p<0, 1, 1>(maxp, 0) : resident_p<0, 0, 1>(maxp, 0);
   }
-  if (decode) return cpass3 ? resident_p<1, 1, 0>(maxp) : resident_p<1, 0, 0>(maxp);
-  return cpass3 ? resident_p<0, 1, 0>(maxp) : resident_p<0, 0, 0>(maxp);
+  if (decode) return cpass3 ? resident_p<1, 1, 0>(maxp, compact) : resident_p<1, 0, 0>(maxp, compact);
+  return cpass3 ? resident_p<0, 1, 0>(maxp, 0) : resident_p<0, 0, 0>(maxp, 0);
 }
 
 template <int DEC, int MAXP, int OCC>

@@ -103,18 +103,33 @@ __global__ __launch_bounds__(256) void queue_keys_kernel(const QueueArgs a) {
     const int tx = cb % im.nx, ty = cb / im.nx;
     const int64_t base = im.layout ? (int64_t)cb * 64 * rows : (int64_t)ty * rows * im.size_x + tx * 64;
     const int rs = im.layout ? 64 : im.size_x;
-    const int32_t* p0 = im.src + base;
     u32 m = 0;
-    if ((((uintptr_t)p0 | ((uintptr_t)rs * 4u)) & 15u) == 0) {
-      // 16 bytes per lane: four rows of 64 coefficients per wave load, loads in flight
-      const int col = (lane & 15) * 4;
+    if (im.in16) {
+      const int16_t* p0 = reinterpret_cast<const int16_t*>(im.src) + base;
+      if ((((uintptr_t)p0 | ((uintptr_t)rs * 2u)) & 7u) == 0) {
+        // 8 bytes per lane: four rows of 64 coefficients per wave load
+        const int col = (lane & 15) * 4;
 #pragma unroll 4
-      for (int r = lane >> 4; r < rows; r += 4) {
-        const int4 v = *reinterpret_cast<const int4*>(p0 + (int64_t)r * rs + col);
-        m |= mag(v.x) | mag(v.y) | mag(v.z) | mag(v.w);
+        for (int r = lane >> 4; r < rows; r += 4) {
+          const short4 v = *reinterpret_cast<const short4*>(p0 + (int64_t)r * rs + col);
+          m |= mag(v.x) | mag(v.y) | mag(v.z) | mag(v.w);
+        }
+      } else {
+        for (int r = 0; r < rows; ++r) m |= mag(p0[(int64_t)r * rs + lane]);
       }
     } else {
-      for (int r = 0; r < rows; ++r) m |= mag(p0[(int64_t)r * rs + lane]);
+      const int32_t* p0 = im.src + base;
+      if ((((uintptr_t)p0 | ((uintptr_t)rs * 4u)) & 15u) == 0) {
+        // 16 bytes per lane: four rows of 64 coefficients per wave load, loads in flight
+        const int col = (lane & 15) * 4;
+#pragma unroll 4
+        for (int r = lane >> 4; r < rows; r += 4) {
+          const int4 v = *reinterpret_cast<const int4*>(p0 + (int64_t)r * rs + col);
+          m |= mag(v.x) | mag(v.y) | mag(v.z) | mag(v.w);
+        }
+      } else {
+        for (int r = 0; r < rows; ++r) m |= mag(p0[(int64_t)r * rs + lane]);
+      }
     }
     m = wave_or32(m);
     const int msb = m ? 31 - __clz(m) : 32;

@@ -247,6 +247,7 @@ int launch_batch(bpc_server* s, Slot& sl, std::vector<Req*>& reqs) {
     im.size_y = q->r.size_y;
     im.params = q->r.params;
     im.fill_unused = 1;
+    im.int16_coeffs = q->r.int16_coeffs;
   }
   // Replayed graph (one per direction and bitplane count: the run's shape is read
   // on the device), status into the slot's pinned word, 28-plane pass only when
@@ -399,7 +400,7 @@ int bpc_server_submit(bpc_server* s, const bpc_request* r, uint64_t* id) {
   if (r->lut_len < bpc_lut_len(&r->params))
     return bpc::fail(BPC_ERR_ARG, "lut_len %d < %d required", r->lut_len, bpc_lut_len(&r->params));
   const size_t out_bytes = r->direction == BPC_CODE ? (size_t)bpc_stream_words(r->size_x, r->size_y, &r->params) * 4
-                                                    : (size_t)r->size_x * r->size_y * 4;
+                                                    : (size_t)r->size_x * r->size_y * (r->int16_coeffs ? 2 : 4);
   if (!s->rule.possible(out_bytes, s->total_mem))
     return bpc::fail(BPC_ERR_ARG, "output of %zu bytes can never be admitted (budget %zu, reserve %zu, device %zu)",
                      out_bytes, s->rule.budget, s->rule.reserve, s->total_mem);

@@ -174,6 +174,15 @@ int bpc_compact_pack_device(const int32_t* d_stream, int64_t ncb, int stride_wor
 int bpc_compact_unpack_device(const uint8_t* d_msb, const uint64_t* d_offsets, const uint16_t* d_payload,
                               uint64_t payload_len, int64_t ncb, int stride_words, int32_t* d_stream,
                               uint32_t* d_status, void* stream);
+/* Decode straight from the compact form into size_x*size_y int32 coefficients in
+ * p->layout (no regions in between): the result, including errors, is that of
+ * bpc_compact_unpack_device with p's region size followed by bpc_decode_device.
+ * d_msb and d_offsets cover the image's codeblocks; d_payload may be NULL when
+ * payload_len is 0.  The counterpart of the reference decoder's region reads
+ * (BPC_PaCo.cu:1216 MSB, :304-316 codewords), with the regions left out. */
+int bpc_decode_compact_device(const uint8_t* d_msb, const uint64_t* d_offsets, const uint16_t* d_payload,
+                              uint64_t payload_len, int32_t* d_coeffs, int size_x, int size_y, const int32_t* d_lut,
+                              int lut_len, const bpc_params* p, uint32_t* d_status, void* stream);
 
 /* ---- per-codeblock codestream files (main_example.cu:51-179) -------------------
  * File k.bin belongs to tile tile_of_file[k] in load_Data's order (LL band, then
@@ -218,6 +227,8 @@ typedef struct bpc_image {
   int32_t size_x, size_y;
   bpc_params params;  /* per image; cblock_length must be the same for every image of a batch */
   int32_t fill_unused; /* encoder: -1 in unused region words */
+  int32_t int16_coeffs; /* 1: the coefficients (encoder src, decoder dst) are int16_t, as for
+                           bpc_encode_device_i16 / bpc_decode_device_i16; 0: int32_t */
 } bpc_image;
 
 typedef struct bpc_batch bpc_batch;
@@ -236,12 +247,13 @@ int bpc_batch_set_vgpr_cap(bpc_batch* batch, int waves_per_simd);
 /* Encodes (BPC_CODE) or decodes (BPC_DECODE) all images, asynchronously on stream.
  * Codeblocks whose MSB needs more bitplanes than the images' lut_n_bitplanes
  * provide for are coded by a second, 28-plane pass, so every legal codeblock is
- * coded.  use_graph != 0 captures the launch sequence into a hipGraph the first
- * time a batch signature (geometry and parameters of every image, codeblock
- * counts, tuning) is seen and replays it afterwards; at most 32 graphs are kept
- * per batch object, the least recently used is dropped.  Buffer pointers, LUTs
- * and d_status are uploaded per run, so they may change between replays.
- * Errors land in d_status (bpc_status_take). */
+ * coded.  use_graph != 0 replays a captured hipGraph of the launch sequence: the
+ * kernels read the run's shape (image count, geometry, parameters, codeblock
+ * counts, buffer pointers, LUTs, d_status) from words uploaded per run, so one
+ * graph per (direction, bitplane count of the widest image, tuning) serves
+ * batches of any composition; it is captured on first use, at most 32 are kept
+ * per batch object and the least recently used is dropped.  Errors land in
+ * d_status (bpc_status_take). */
 int bpc_batch_run(bpc_batch* batch, int direction, const bpc_image* images, int n_images, int use_graph,
                   uint32_t* d_status, void* stream);
 /* Number of captured graphs the batch object holds. */
@@ -280,6 +292,7 @@ typedef struct bpc_request {
   int32_t lut_len;
   int32_t size_x, size_y;
   bpc_params params;
+  int32_t int16_coeffs; /* 1: coefficients are int16_t (encode input, decode output), else int32_t */
 } bpc_request;
 
 typedef struct bpc_result {

@@ -27,7 +27,7 @@ int main(void) {
   BEGIN(bpc_image);
   FIELD(bpc_image, src); FIELD(bpc_image, dst); FIELD(bpc_image, lut); FIELD(bpc_image, lut_len);
   FIELD(bpc_image, ncw); FIELD(bpc_image, size_x); FIELD(bpc_image, size_y); FIELD(bpc_image, params);
-  FIELD(bpc_image, fill_unused);
+  FIELD(bpc_image, fill_unused); FIELD(bpc_image, int16_coeffs);
   END();
   BEGIN(bpc_server_config);
   FIELD(bpc_server_config, max_inflight); FIELD(bpc_server_config, max_batch_images);
@@ -39,6 +39,7 @@ int main(void) {
   BEGIN(bpc_request);
   FIELD(bpc_request, direction); FIELD(bpc_request, src); FIELD(bpc_request, lut); FIELD(bpc_request, lut_len);
   FIELD(bpc_request, size_x); FIELD(bpc_request, size_y); FIELD(bpc_request, params);
+  FIELD(bpc_request, int16_coeffs);
   END();
   BEGIN(bpc_result);
   FIELD(bpc_result, status); FIELD(bpc_result, output); FIELD(bpc_result, ncw); FIELD(bpc_result, queue_us);

@@ -294,3 +294,49 @@ def test_batch_graph_replay_folds_into_the_current_status_word(bpc):
         free_set(items)
         st1.free()
         st2.free()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_int16_and_int32_images_vs_oracle(bpc, O, graph):
+    """A batch mixing int16 and int32 coefficient images (raster and tiled, 2- and
+    3-pass): every codestream and count as the oracle's, every int16 reconstruction
+    equal to the original."""
+    specs = [(512, 0, 15, 1, 64), (768, 1, 15, 0, 64), (1024, 1, 16, 1, 64), (512, 1, 15, 0, 64), (256, 0, 14, 1, 64)]
+    items = make_set(bpc, specs, seed=9400, mag=14)
+    wide16 = {0, 2, 4}
+    buf16 = {}
+    try:
+        for k in wide16:
+            it = items[k]
+            src = it["img"] if it["p"].layout == 0 else synth.to_tiled(it["img"], it["p"].cblock_length)
+            buf16[k] = (bpc.DeviceBuffer.from_array(np.ascontiguousarray(src).astype(np.int16).ravel()),
+                        bpc.DeviceBuffer(it["size"] ** 2, np.int16))
+        batch = bpc.Batch(8, 1 << 14)
+        try:
+            enc = [bpc.make_image(buf16[k][0] if k in buf16 else it["d_in"], it["d_cs"], it["lut"], it["size"],
+                                  it["size"], it["p"], it["d_ncw"], int16=k in buf16) for k, it in enumerate(items)]
+            dec = [bpc.make_image(it["d_cs"], buf16[k][1] if k in buf16 else it["d_out"], it["lut"], it["size"],
+                                  it["size"], it["p"], int16=k in buf16) for k, it in enumerate(items)]
+            for it in items:
+                it["d_cs"].fill_bytes(0x21)
+            batch.run(bpc.CODE, enc, use_graph=graph)
+            batch.run(bpc.DECODE, dec, use_graph=graph)
+            assert bpc.status_take() == 0, bpc.lib().bpc_last_error()
+        finally:
+            batch.destroy()
+        for k, it in enumerate(items):
+            p, img = it["p"], it["img"]
+            op = O.params(cpass3=p.cpass3, dwt_levels=p.dwt_levels, cblock_length=p.cblock_length,
+                          lut_n_bitplanes=p.lut_n_bitplanes)
+            st, ostream, oncw = O.encode(img, it["lut_arr"], op)
+            assert st == 0
+            assert np.array_equal(it["d_cs"].download(), ostream), k
+            assert np.array_equal(it["d_ncw"].download(), oncw), k
+            got = (buf16[k][1] if k in buf16 else it["d_out"]).download().astype(np.int32)
+            got = got.reshape(img.shape) if p.layout == 0 else synth.from_tiled(got, it["size"], it["size"])
+            assert np.array_equal(got, img), k
+    finally:
+        for a, b in buf16.values():
+            a.free()
+            b.free()
+        free_set(items)

@@ -110,3 +110,85 @@ def test_bin_files_from_device(bpc, O, tmp_path):
     assert np.array_equal(bpc.bin_import(str(tmp_path), 2048, 2048, p), ostream)
     c.free()
     d_cs.free()
+
+
+@pytest.mark.parametrize("size,cp3,nbp,rows,stride,layout", [
+    (2048, 0, 15, 64, 0, 0), (1024, 1, 16, 32, 0, 1), (1024, 1, 15, 64, 6002, 0), (4096, 1, 16, 64, 0, 1),
+    (256, 0, 15, 64, 0, 0)])
+def test_decode_compact_matches_oracle(bpc, O, size, cp3, nbp, rows, stride, layout):
+    """Decoding straight from the compact form of the oracle's codestream gives the
+    original coefficients (the per-image launch with and without register planes:
+    256^2 and 1024^2 fit every wave at once, 4096^2 grid-strides over slots)."""
+    print("decode_compact", size, cp3, nbp, rows, stride, layout, flush=True)
+    img, p, lut_arr, ostream, oncw = encoded(bpc, O, size, cp3, nbp, 170 + size // 256 + cp3, rows, stride,
+                                             mag_bits=16 if nbp >= 16 else 14)
+    p.layout = layout
+    ncb, sw = p.n_codeblocks(size, size), p.stride()
+    lut = bpc.Lut.from_array(lut_arr)
+    d_cs = bpc.DeviceBuffer.from_array(ostream)
+    d_rec = bpc.DeviceBuffer(size * size)
+    try:
+        c = bpc.compact_pack(d_cs, ncb, sw)
+        assert bpc.status_take() == 0
+        d_rec.fill_bytes(0x77)
+        bpc.decode_compact_device(c, d_rec, size, size, lut, p)
+        assert bpc.status_take() == 0
+        got = d_rec.download()
+        want = synth.to_tiled(img, rows).ravel() if layout else img.ravel()
+        assert np.array_equal(got, want)
+        c.free()
+    finally:
+        d_cs.free()
+        d_rec.free()
+        lut.free()
+
+
+def test_decode_compact_equals_unpack_then_decode_on_damaged_input(bpc, O):
+    """Damaged containers (a malformed MSB byte, counts past the region, a payload
+    shorter than the offsets say, decreasing offsets) decode exactly as the
+    regions that unpack makes of them: same coefficients, same status."""
+    size, rows = 512, 64
+    img, p, lut_arr, ostream, oncw = encoded(bpc, O, size, 1, 15, 180, rows)
+    ncb, sw = p.n_codeblocks(size, size), p.stride()
+    lut = bpc.Lut.from_array(lut_arr)
+    d_cs = bpc.DeviceBuffer.from_array(ostream)
+    c = bpc.compact_pack(d_cs, ncb, sw)
+    assert bpc.status_take() == 0
+    msb, offsets, payload = c.download()
+    rng = np.random.default_rng(5)
+    cases = []
+    m = msb.copy(); m[3] = 255; cases.append(("malformed msb", m, offsets, payload, c.total))
+    o = offsets.copy(); o[6:] += sw; cases.append(("count past region", msb, o, payload, c.total))
+    cases.append(("short payload", msb, offsets, payload, c.total - 500))
+    o = offsets.copy(); o[9] = o[11]; cases.append(("decreasing offsets", msb, o, payload, c.total))
+    pl = payload.copy(); idx = rng.integers(0, c.total, 40); pl[idx] ^= 0x5A5A
+    cases.append(("flipped codewords", msb, offsets, pl, c.total))
+    d_regions = bpc.DeviceBuffer(ncb * sw)
+    d_a = bpc.DeviceBuffer(size * size)
+    d_b = bpc.DeviceBuffer(size * size)
+    try:
+        for name, m, o, pl, total in cases:
+            print("damaged:", name, flush=True)
+            dc = bpc.Compact(bpc.DeviceBuffer.from_array(m), bpc.DeviceBuffer.from_array(o),
+                             bpc.DeviceBuffer.from_array(pl), ncb, int(total))
+            try:
+                bpc.compact_unpack(dc, d_regions, sw)
+                st_unpack = bpc.status_take()
+                bpc.decode_device(d_regions, d_a, size, size, lut, p)
+                st_a = bpc.status_take()
+                bpc.decode_compact_device(dc, d_b, size, size, lut, p)
+                st_b = bpc.status_take()
+                assert np.array_equal(d_a.download(), d_b.download()), name
+                # one status word for both steps: the first of their codes in the
+                # order bpc_status_take reports them
+                order = [bpc.ERR_MALFORMED, bpc.ERR_RANGE, bpc.ERR_OVERFLOW, bpc.ERR_WIDE, 0]
+                want = min(st_unpack, st_a, key=order.index)
+                assert st_b == want, (name, st_unpack, st_a, st_b)
+                assert name == "flipped codewords" or want != 0, name  # every other case is damaged
+            finally:
+                dc.free()
+    finally:
+        for b in (d_cs, d_regions, d_a, d_b):
+            b.free()
+        c.free()
+        lut.free()

@@ -185,3 +185,38 @@ def test_server_rejects_oversized_request(bpc):
         lut.free()
     finally:
         srv.destroy()
+
+
+def test_server_int16_requests_vs_oracle(bpc, O):
+    """int16 encode and decode requests served in the same batches as int32 ones:
+    codestreams and counts as the oracle's, int16 outputs equal to the originals."""
+    pool = make_pool(bpc, O, [(512, 0, 15), (1024, 1, 15), (768, 1, 14)], seed=5600)
+    for it in pool:
+        it["d_img16"] = bpc.DeviceBuffer.from_array(it["rec"].astype(np.int16).ravel())
+    srv = bpc.Server(max_inflight=2, max_batch_images=8)
+    rng = random.Random(3)
+    plan = [(rng.randrange(len(pool)), rng.random() < 0.5, rng.random() < 0.6) for _ in range(30)]
+    try:
+        ids = []
+        for j, enc, i16 in plan:
+            it = pool[j]
+            src = (it["d_img16"] if i16 else it["d_img"]) if enc else it["d_cs"]
+            ids.append(srv.submit(bpc.CODE if enc else bpc.DECODE, src, it["lut"], it["size"], it["size"], it["p"],
+                                  int16=i16))
+        for rid, (j, enc, i16) in zip(ids, plan):
+            it = pool[j]
+            res = srv.wait(rid, timeout_ms=60000)
+            assert res.status == 0
+            if enc:
+                assert np.array_equal(bpc.download_ptr(res.output, it["cs"].size), it["cs"])
+                ncb = it["p"].n_codeblocks(it["size"], it["size"])
+                assert np.array_equal(bpc.download_ptr(res.ncw, ncb), it["ncw"])
+            else:
+                got = bpc.download_ptr(res.output, it["size"] ** 2, np.int16 if i16 else np.int32)
+                assert np.array_equal(got.astype(np.int32), it["rec"].ravel())
+            srv.release(rid)
+    finally:
+        srv.destroy()
+        for it in pool:
+            it["d_img16"].free()
+        free_pool(pool)
