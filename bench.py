@@ -538,12 +538,14 @@ def bench_large(args, check=True):
     return out
 
 
-def bench_container(args, d_cs, d_ncw, ncb, stride, stream):
+def bench_container(args, d_cs, d_ncw, ncb, stride, stream, plain):
     """Compact container of the plain workload's codestream: index + pack (regions ->
     MSB bytes, offsets, uint16 codewords) and unpack (back to -1-filled regions), with
     the round trip checked.  HBM-bound; algorithmic bytes per codeblock: pack reads
     the MSB, count and used words (4 * (2 + n)) and writes 1 + 8 + 2n; unpack reads
-    1 + 8 + 2n and writes the whole region (4 * stride)."""
+    1 + 8 + 2n and writes the whole region (4 * stride).  Then the decoder reading the
+    compact form directly against the region decoder (and unpack + decode), both
+    checked against the original coefficients.  plain: (lut, params, size, d_in)."""
     msb = bpc.DeviceBuffer(ncb, np.uint8)
     off = bpc.DeviceBuffer(ncb + 1, np.uint64)
     bpc.compact_index(d_cs, ncb, stride, msb, off, d_ncw=d_ncw, stream=stream)
@@ -571,7 +573,28 @@ def bench_container(args, d_cs, d_ncw, ncb, stride, stream):
     pack_ms, unpack_ms = float(np.median(tp)), float(np.median(tu))
     pack_bytes = 4.0 * (2 * ncb + total) + 9.0 * ncb + 2.0 * total
     unpack_bytes = 9.0 * ncb + 2.0 * total + 4.0 * stride * ncb
-    for b in (payload, back, msb, off):
+    # decoding: from the regions, and straight from the compact form
+    lut, p, size, d_in = plain
+    rec_r, rec_c = bpc.DeviceBuffer(size * size), bpc.DeviceBuffer(size * size)
+    tr, tc = [], []
+    for k in range(max(1, args.warmup) + max(5, args.steps)):
+        ev[0].record(stream)
+        bpc.decode_device(d_cs, rec_r, size, size, lut, p, stream=stream)
+        ev[1].record(stream)
+        bpc.decode_compact_device(c, rec_c, size, size, lut, p, stream=stream)
+        ev[2].record(stream)
+        if k >= max(1, args.warmup):
+            tr.append(ev[0].elapsed_ms(ev[1]))
+            tc.append(ev[1].elapsed_ms(ev[2]))
+    if bpc.status_take(None, stream) != 0:
+        raise RuntimeError("decode status: %s" % bpc.lib().bpc_last_error().decode())
+    orig = d_in.download(stream=stream)
+    dec_exact = bool(np.array_equal(rec_c.download(stream=stream), orig) and
+                     np.array_equal(rec_r.download(stream=stream), orig))
+    if not dec_exact:
+        raise RuntimeError("decode from the compact form differs from the original")
+    dec_r_ms, dec_c_ms = float(np.median(tr)), float(np.median(tc))
+    for b in (payload, back, msb, off, rec_r, rec_c):
         b.free()
     return {"workload": "compact container of the plain_4096_3pass codestream",
             "compact_bytes": c.nbytes, "reference_layout_bytes": 4 * stride * ncb,
@@ -579,7 +602,9 @@ def bench_container(args, d_cs, d_ncw, ncb, stride, stream):
             "pack_ms": round(pack_ms, 4), "unpack_ms": round(unpack_ms, 4),
             "pack_gbs": round(pack_bytes / pack_ms / 1e6, 1), "unpack_gbs": round(unpack_bytes / unpack_ms / 1e6, 1),
             "unpack_hbm_frac": round(unpack_bytes / unpack_ms / 1e6 / HBM_PEAK_GBS, 4),
-            "round_trip_exact": exact}
+            "round_trip_exact": exact,
+            "decode_regions_ms": round(dec_r_ms, 4), "decode_compact_ms": round(dec_c_ms, 4),
+            "unpack_then_decode_ms": round(unpack_ms + dec_r_ms, 4), "decode_compact_exact": dec_exact}
 
 
 def bench_dwt(args, check=True):
@@ -942,7 +967,7 @@ def main():
             raise RuntimeError("GPU codestream differs from the oracle")
     wl = args.workloads.split(",")
     if "container" in wl:
-        out["container"] = bench_container(args, d_cs, d_ncw, ncb, p.stride(), stream)
+        out["container"] = bench_container(args, d_cs, d_ncw, ncb, p.stride(), stream, (lut, p, size, d_in))
     for b in (d_in, d_cs, d_out, d_ncw, d_status):
         b.free()
     lut.free()

@@ -340,3 +340,31 @@ def test_batch_int16_and_int32_images_vs_oracle(bpc, O, graph):
             a.free()
             b.free()
         free_set(items)
+
+
+def test_batch_int16_decode_saturates(bpc, O):
+    """In a batch, an int16 decode of magnitudes past the int16 range saturates and
+    reports BPC_ERR_RANGE, while an int32 image of the same batch decodes exactly."""
+    size = 256
+    img = synth.synth_image(size, size, 3, 211, ll_bits=17, mag_bits=17)
+    assert (np.abs(img.astype(np.int64)) > 32767).any()
+    p = bpc.params(cpass3=1, dwt_levels=3, lut_n_bitplanes=18)
+    lut_arr = bpc.train_lut(img, p)
+    st, ostream, _ = O.encode(img, lut_arr, O.params(cpass3=1, dwt_levels=3, lut_n_bitplanes=18))
+    assert st == 0
+    lut = bpc.Lut.from_array(lut_arr)
+    cs = bpc.DeviceBuffer.from_array(ostream)
+    out16 = bpc.DeviceBuffer(size * size, np.int16)
+    out32 = bpc.DeviceBuffer(size * size)
+    batch = bpc.Batch(2, 1 << 10)
+    try:
+        dec = [bpc.make_image(cs, out16, lut, size, size, p, int16=True), bpc.make_image(cs, out32, lut, size, size, p)]
+        batch.run(bpc.DECODE, dec)
+        assert bpc.status_take() == bpc.ERR_RANGE
+        assert np.array_equal(out16.download().reshape(size, size), np.clip(img, -32768, 32767).astype(np.int16))
+        assert np.array_equal(out32.download().reshape(size, size), img)
+    finally:
+        batch.destroy()
+        for b in (cs, out16, out32):
+            b.free()
+        lut.free()

@@ -10,12 +10,12 @@ from test_container_host import compact_of
 pytestmark = pytest.mark.gpu
 
 
-def encoded(bpc, O, size, cp3, nbp, seed, rows=64, stride=0, mag_bits=14, zero_cb=True):
-    img = synth.synth_image(size, size, 5, seed, mag_bits=mag_bits)
+def encoded(bpc, O, size, cp3, nbp, seed, rows=64, stride=0, mag_bits=14, zero_cb=True, ll_bits=13):
+    img = synth.synth_image(size, size, 5, seed, ll_bits=ll_bits, mag_bits=mag_bits)
     if zero_cb:
         img[0:rows, 64:128] = 0
     p = bpc.params(cpass3=cp3, cblock_length=rows, lut_n_bitplanes=nbp, cb_stride_words=stride)
-    lut_arr = bpc.train_lut(synth.synth_image(size, size, 5, seed + 1, mag_bits=mag_bits), p)
+    lut_arr = bpc.train_lut(synth.synth_image(size, size, 5, seed + 1, ll_bits=ll_bits, mag_bits=mag_bits), p)
     st, ostream, oncw = O.encode(img, lut_arr, O.params(cpass3=cp3, cblock_length=rows, lut_n_bitplanes=nbp,
                                                         cb_stride_words=stride))
     assert st == 0
@@ -114,14 +114,16 @@ def test_bin_files_from_device(bpc, O, tmp_path):
 
 @pytest.mark.parametrize("size,cp3,nbp,rows,stride,layout", [
     (2048, 0, 15, 64, 0, 0), (1024, 1, 16, 32, 0, 1), (1024, 1, 15, 64, 6002, 0), (4096, 1, 16, 64, 0, 1),
-    (256, 0, 15, 64, 0, 0)])
+    (256, 0, 15, 64, 0, 0), (1024, 1, 20, 64, 8192, 1), (512, 0, 24, 64, 12288, 0)])
 def test_decode_compact_matches_oracle(bpc, O, size, cp3, nbp, rows, stride, layout):
     """Decoding straight from the compact form of the oracle's codestream gives the
     original coefficients (the per-image launch with and without register planes:
-    256^2 and 1024^2 fit every wave at once, 4096^2 grid-strides over slots)."""
+    256^2 and 1024^2 fit every wave at once, 4096^2 grid-strides over slots; 20 and
+    24 LUT bitplanes take the 20- and 28-plane kernels, with regions wide enough)."""
     print("decode_compact", size, cp3, nbp, rows, stride, layout, flush=True)
     img, p, lut_arr, ostream, oncw = encoded(bpc, O, size, cp3, nbp, 170 + size // 256 + cp3, rows, stride,
-                                             mag_bits=16 if nbp >= 16 else 14)
+                                             mag_bits=min(nbp, 22) if nbp >= 16 else 14,
+                                             ll_bits=min(nbp, 22) - 1 if nbp > 16 else 13)
     p.layout = layout
     ncb, sw = p.n_codeblocks(size, size), p.stride()
     lut = bpc.Lut.from_array(lut_arr)
