@@ -161,11 +161,9 @@ int launch(int decode, const int32_t* src, int32_t* dst, int size_x, int size_y,
   a.in16 = in16;
   const int compact = decode && cmp ? 1 : 0;
   if (compact) {
-    a.cmsb = cmp->msb;
+    a.cmsb = cmp->msb;  // non-null: selects the decoder that reads the compact form
     a.coff = reinterpret_cast<const unsigned long long*>(cmp->offsets);
-    // a non-null payload selects the compact variant, also for an empty payload
-    static const uint16_t kNoPayload = 0;
-    a.cpay = cmp->payload ? cmp->payload : &kNoPayload;
+    a.cpay = cmp->payload;  // null only with payload_len 0, when no count can pass the bounds test
     a.ccap = cmp->payload_len;
   }
   if (a.ncb <= 0) return BPC_OK;

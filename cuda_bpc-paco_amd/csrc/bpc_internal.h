@@ -29,8 +29,9 @@ struct KArgs {
   int in16;            // coefficients are int16 (encoder input / decoder output), else int32
   void* scratch;       // per wave of the grid: 2 * MAXP * 64 uint2 (scratch_bytes_per_wave); unused
                        // by the variant with the bitplanes in registers
-  // decoder only, all null unless it reads the compact container instead of src:
-  // MSB byte per codeblock, offsets (ncb + 1), payload of ccap codewords
+  // decoder only, all null unless it reads the compact container instead of src
+  // (selected by cmsb): MSB byte per codeblock, offsets (ncb + 1), payload of ccap
+  // codewords (null when ccap is 0)
   const uint8_t* cmsb;
   const unsigned long long* coff;
   const uint16_t* cpay;

@@ -908,7 +908,7 @@ __device__ __forceinline__ void code_pair(const HalfDesc& a, const int rows, uin
         status |= ST_OVERFLOW;
         n = 0;
       }
-      ccw.pay = a.cpay + o;
+      ccw.pay = n ? a.cpay + o : nullptr;  // read only below slot n + 1
       ccw.n = (u32)n;
     }
     msb = valid ? (int)a.cmsb[cbc] : 32;  // 255 marks a malformed MSB word
@@ -1347,7 +1347,7 @@ int planes_for(int nbp) {
 // The compact-container variants exist for the decoder only.
 template <int DEC, int CP3, int MAXP, int RP>
 static hipError_t launch_one(const KArgs& a, int grid_waves, hipStream_t stream) {
-  if (DEC && a.cpay)
+  if (DEC && a.cmsb)
     hipLaunchKernelGGL((bpc_codec_kernel<DEC, CP3, MAXP, RP, DEC>), dim3(grid_waves), dim3(64), 0, stream, a);
   else
     hipLaunchKernelGGL((bpc_codec_kernel<DEC, CP3, MAXP, RP>), dim3(grid_waves), dim3(64), 0, stream, a);

@@ -194,3 +194,32 @@ def test_decode_compact_equals_unpack_then_decode_on_damaged_input(bpc, O):
             b.free()
         c.free()
         lut.free()
+
+
+def test_decode_compact_all_zero_image_without_payload(bpc):
+    """An all-zero image has MSB 32 everywhere and no codewords: its container has
+    an empty payload, which may be passed as a null pointer; it decodes to zeros."""
+    size = 512
+    p = bpc.params(cpass3=1, lut_n_bitplanes=15)
+    ncb, sw = p.n_codeblocks(size, size), p.stride()
+    lut = bpc.Lut.from_array(np.full(bpc.lut_len(p), 64, np.int32))
+    d_img = bpc.DeviceBuffer.from_array(np.zeros(size * size, np.int32))
+    d_cs = bpc.DeviceBuffer(ncb * sw)
+    d_rec = bpc.DeviceBuffer(size * size)
+    try:
+        bpc.encode_device(d_img, d_cs, size, size, lut, p)
+        assert bpc.status_take() == 0
+        c = bpc.compact_pack(d_cs, ncb, sw)
+        assert bpc.status_take() == 0 and c.total == 0
+        msb, offsets, _ = c.download()
+        assert (msb == 32).all() and (offsets == 0).all()
+        d_rec.fill_bytes(0x3C)
+        empty = bpc.Compact(c.msb, c.offsets, None, ncb, 0)
+        bpc.decode_compact_device(empty, d_rec, size, size, lut, p)
+        assert bpc.status_take() == 0
+        assert (d_rec.download() == 0).all()
+        c.free()
+    finally:
+        for b in (d_img, d_cs, d_rec):
+            b.free()
+        lut.free()
