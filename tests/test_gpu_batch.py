@@ -368,3 +368,55 @@ def test_batch_int16_decode_saturates(bpc, O):
         for b in (cs, out16, out32):
             b.free()
         lut.free()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_at_image_capacity_vs_oracle(bpc, O, graph):
+    """4096 small images in one batch (the capacity): the staging area read by the
+    run's first kernel at its largest, one- and two-codeblock images (odd counts
+    leave padding entries), 2- and 3-pass alternating; every codestream and
+    reconstruction against the oracle."""
+    n = bpc.MAX_BATCH_IMAGES if hasattr(bpc, "MAX_BATCH_IMAGES") else 4096
+    rng = np.random.default_rng(77)
+    shapes = [(64, 64), (128, 64)]
+    luts = {}
+    for cp3 in (0, 1):
+        train = synth.synth_image(128, 64, 1, 7700 + cp3, mag_bits=12)
+        luts[cp3] = bpc.train_lut(train, bpc.params(cpass3=cp3, dwt_levels=1, lut_n_bitplanes=13))
+    lut_dev = {cp3: bpc.Lut.from_array(a) for cp3, a in luts.items()}
+    items = []
+    try:
+        for k in range(n):
+            sx, sy = shapes[k % 2]
+            cp3 = (k // 2) % 2
+            img = synth.synth_image(sx, sy, 1, 8000 + k, mag_bits=12) if k % 7 else np.zeros((sy, sx), np.int32)
+            p = bpc.params(cpass3=cp3, dwt_levels=1, lut_n_bitplanes=13)
+            ncb = p.n_codeblocks(sx, sy)
+            items.append(dict(img=img, sx=sx, sy=sy, cp3=cp3, p=p, d_in=bpc.DeviceBuffer.from_array(img.ravel()),
+                              d_cs=bpc.DeviceBuffer(bpc.stream_words(sx, sy, p)), d_out=bpc.DeviceBuffer(sx * sy),
+                              d_ncw=bpc.DeviceBuffer(ncb)))
+        total = sum(it["p"].n_codeblocks(it["sx"], it["sy"]) for it in items)
+        batch = bpc.Batch(n, total)
+        try:
+            enc = [bpc.make_image(it["d_in"], it["d_cs"], lut_dev[it["cp3"]], it["sx"], it["sy"], it["p"], it["d_ncw"])
+                   for it in items]
+            dec = [bpc.make_image(it["d_cs"], it["d_out"], lut_dev[it["cp3"]], it["sx"], it["sy"], it["p"])
+                   for it in items]
+            batch.run(bpc.CODE, enc, use_graph=graph)
+            batch.run(bpc.DECODE, dec, use_graph=graph)
+            assert bpc.status_take() == 0, bpc.lib().bpc_last_error()
+        finally:
+            batch.destroy()
+        for k, it in enumerate(items):
+            st, ostream, oncw = O.encode(it["img"], luts[it["cp3"]],
+                                         O.params(cpass3=it["cp3"], dwt_levels=1, lut_n_bitplanes=13))
+            assert st == 0
+            assert np.array_equal(it["d_cs"].download(), ostream), k
+            assert np.array_equal(it["d_ncw"].download(), oncw), k
+            assert np.array_equal(it["d_out"].download().reshape(it["sy"], it["sx"]), it["img"]), k
+    finally:
+        for it in items:
+            for key in ("d_in", "d_cs", "d_out", "d_ncw"):
+                it[key].free()
+        for l in lut_dev.values():
+            l.free()
